@@ -1,0 +1,159 @@
+"""Benchmark: particle-steps/s of the BPM uniaxial loading step on one MI355X.
+
+Workload (default): 100,000 polydisperse spheres (r ~ U[0.20, 0.35] mm) in a
+2:1 cylinder, bonded (gap < 0.05 r_min, bond E 20 GPa, lognormal strengths
+40/60 MPa CV 0.2), Hertz-Mindlin contacts (E 50 GPa, nu 0.25, mu 0.5),
+platens at 0.1/s strain rate, FP64.  A "step" is one full loading step:
+list maintenance, bonds, contacts, platens, kick-drift, cut and the device
+controller.
+
+Prints one JSON line.  `value` is whole-job particle-steps/s with all state
+resident in HBM; `roofline` prices the fused step kernel's algorithmic bytes
+against the 8 TB/s HBM peak; `cpu_baseline` times the FP64 CPU oracle on a
+bounded number of steps of the same workload on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dem-rock-calibration_amd"))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def device_sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed ({rc})")
+
+
+def step_kernel_bytes(c):
+    """Algorithmic HBM bytes of one fused step (FP64): particle state R, V, W
+    read and written (144 B/particle), slot index + kind of each occupied
+    directed slot (5 B), per directed bond 1/L0 plus prev-normal, shear,
+    torsion and bending histories read and written (200 B), per overlapping
+    contact its tangential overlap read and written (48 B)."""
+    return 144 * c.n + 5 * c.occupied + 200 * c.bonds + 48 * c.contacts
+
+
+def cpu_baseline(w, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    pyoracle.build()
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], threads=threads)
+    o.build_bonds()
+    o.set_test(w["cfg"])
+    o.step(2)  # untimed warm-up
+    t0 = time.perf_counter()
+    k = o.step(steps)
+    dt = time.perf_counter() - t0
+    return dict(value=w["n"] * k / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"{k} loading steps of the same {w['n']}-particle uniaxial workload "
+                       f"after 2 warm-up steps, FP64 CPU oracle, OpenMP {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--cpu-steps", type=int, default=-1,
+                    help="CPU-baseline steps (default: sized to ~15 s of CPU work)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(a sample shards by slab decomposition only in design, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from rockfit import config
+    from rockfit.core import Context
+
+    t0 = time.perf_counter()
+    w = config.make_workload(n=args.n, r_min=0.20, r_max=0.35, test="uniaxial")
+    # strain rate 0.1/s: closing speed = 0.1/s * specimen height
+    height_mm = w["cfg"].size[2]
+    v_mps = -0.1 * height_mm * 1e-3
+    w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v_mps, dnsave=100)
+    log(f"sample n={w['n']} generated in {time.perf_counter() - t0:.1f}s, "
+        f"height {height_mm:.2f} mm, v_load {v_mps:.3e} m/s")
+
+    ctx = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    ctx.build_bonds()
+    ctx.set_test(w["cfg"])
+    c0 = ctx.counters()
+    log(f"built: partners/particle {c0.occupied / c0.n:.2f}, bonds/particle "
+        f"{c0.bonds / c0.n:.2f}, candidates/particle {c0.candidates / c0.n:.2f}")
+    ctx.set_chunk(256)
+    done = ctx.step(args.warmup)
+    device_sync()
+    log(f"warm-up {done} steps")
+
+    ctx.set_timing(True)
+    device_sync()
+    t1 = time.perf_counter()
+    done = ctx.step(args.steps)
+    device_sync()
+    wall = time.perf_counter() - t1
+    kernel_ms, total_ms, steps = ctx.last_timing()
+    c = ctx.counters()
+    if done != args.steps:
+        log(f"WARNING: test ended after {done} of {args.steps} timed steps")
+    ms_per_step = 1e3 * wall / max(done, 1)
+    value = c.n * done / wall
+    bytes_step = step_kernel_bytes(c)
+    kern_avg_ms = kernel_ms / max(steps, 1)
+    achieved = bytes_step / (kern_avg_ms * 1e-3) / 1e9
+    log(f"timed {done} steps: wall {wall:.3f}s, step kernel avg {kern_avg_ms * 1e3:.1f} us, "
+        f"{bytes_step / c.n:.0f} B/particle-step, rebuilds {c.rebuilds}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ksteps = args.cpu_steps
+        if ksteps < 0:
+            ksteps = max(5, int(15.0 * 5e5 * threads / c.n))
+        log(f"cpu baseline: {ksteps} steps on {threads} threads")
+        cpu = cpu_baseline(w, ksteps, threads)
+
+    out = {
+        "metric": "particle-steps/s at 100k-particle BPM uniaxial test (1 GPU)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": done,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator: jittered-FCC start, overlap relaxation)",
+        "config": {"workload": f"BPM uniaxial compression, {c.n} spheres r~U[0.20,0.35] mm, "
+                               f"2:1 cylinder, strain rate 0.1/s, FP64",
+                   "n_particles": c.n, "partners_per_particle": c.occupied / c.n,
+                   "bonds_per_particle": c.bonds / c.n,
+                   "contacts_per_particle": c.contacts / c.n,
+                   "step_kernel_us": kern_avg_ms * 1e3, "rebuilds": c.rebuilds},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,109 @@
+// rf_internal.h -- device data layout and launch interface of the gfx950 DEM
+// step.  Shared by the kernels (rf_kernels.hip) and the C-ABI host runtime
+// (rf_runtime.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rf {
+
+// Particles handled by one 256-thread block of the step kernel: one 32-lane
+// group per particle, eight groups per block.
+constexpr int kGroup = 32;
+constexpr int kBlock = 256;
+constexpr int kPerBlock = kBlock / kGroup;  // 8 particles per block
+constexpr int kMaxSlots = 32;               // slots per particle (one lane each)
+constexpr int kMaxCand = 64;                // candidates per particle (two per lane)
+constexpr int kCtrlThreads = 256;           // threads of the controller block
+
+// Error bits raised by device code (Ctrl::err).
+constexpr int kErrRowOverflow = 1;   // no free slot for a new contact
+constexpr int kErrCandOverflow = 2;  // candidate list longer than cand_cap
+constexpr int kErrGrid = 4;          // particle outside the padded cell grid
+constexpr int kErrCurve = 8;         // curve buffer full
+
+// Device-resident arrays.  Particles are array-of-structs double4 so a
+// neighbour gather is one 32-byte load:
+//   pos = (x, y, z, radius), vel = (vx, vy, vz, mass), angv = (wx, wy, wz, 0).
+// pos/vel/angv are double-buffered; Ctrl::cur selects the input buffer.
+// Per-particle rows are row-major so the 32 lanes of a group read one
+// contiguous run: slots [i*S + k], candidates [i*C + c], bonds [i*B + b].
+struct DevState {
+    double4* pos[2];
+    double4* vel[2];
+    double4* angv[2];
+    double2* pk;       // (dt/m_i, dt/I_i)
+    double4* F;        // last step's force  (x, y, z, 0)
+    double4* M;        // last step's moment
+    double4* rbuild;   // positions at the last candidate rebuild
+    // slot table
+    int32_t* nbr;      // partner of slot k (valid where the occupancy bit is set)
+    double* otx; double* oty; double* otz;  // contact tangential overlap
+    uint32_t* occ;     // occupied slots
+    uint32_t* bnd;     // bond slots
+    uint32_t* otnz;    // contact slots whose o_t may be non-zero
+    // bond table (bonds numbered in ascending slot order at creation)
+    int32_t* nbond;
+    int8_t* bslot; int32_t* bpart;
+    double* binvL0; double* bsc; double* btc;
+    double* bpnx; double* bpny; double* bpnz;
+    double* botx; double* boty; double* botz;
+    double* bmnx; double* bmny; double* bmnz;
+    double* bmtx; double* bmty; double* bmtz;
+    // candidate lists
+    int32_t* cand; int8_t* cslot; int32_t* ncand;
+    // cell sort
+    int32_t* ckey; int32_t* ccount; int32_t* cstart; int32_t* cfill; int32_t* ctmp;
+    int32_t* sorted;
+    // reductions
+    double* part_b; double* part_t; int32_t* part_bonds;
+    // curve
+    double* curve; int64_t* bcount;
+};
+
+// Constant per run, passed by value.
+struct DevParams {
+    int64_t n;
+    int32_t S, C, B;
+    int32_t nblk;        // blocks of the step kernel
+    double dt, E, G, hmE, hmG, pA, mu, mur;
+    double acut2, cell, inv_cell, cand_r2, skin_half2;
+    double b_ratio, b_abs, bond_gap;
+    uint32_t quirks;
+    int32_t nx, ny, nz;
+    int64_t ncell;
+    double cen[3], half[3], hid[3];
+    int32_t kind;        // rf_test_kind
+    int32_t axis;        // platen axis: 2 uniaxial, 1 Brazilian
+    double Ew, Gw;       // wall moduli (factor * hm_E, factor * hm_G)
+    double scope_c[3];
+    double rr_scope, zcut;
+    double v_load, min_stress, max_strain, cond_fracture, zero_stress, inv_area;
+    int32_t dnsave, store_fm;
+    int64_t max_steps, curve_cap;
+};
+
+// Device control block: written by the controller kernel once per step.
+struct Ctrl {
+    double top, bottom, vw, size_axis, size0, strain, stress;
+    double peak_stress, peak_strain, sfl0, sfl1;
+    int64_t step, steps_stop, stepsave, last_bonds;
+    int32_t loading, fracture, done, halt;
+    int32_t cur, rebuild, err, pad;
+};
+
+// Launch wrappers (rf_kernels.hip).
+hipError_t launch_cell_keys(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_cell_sort(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_construct_rows(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_construct_bonds(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_prune_initial(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_build_candidates(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_clear_flags(Ctrl* ctrl, hipStream_t st);
+hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_counters(const DevState& s, const DevParams& p, Ctrl* ctrl, int64_t* out3,
+                           hipStream_t st);
+
+}  // namespace rf

@@ -1,0 +1,902 @@
+// rf_runtime.hip -- host side of the C-ABI (include/rockfit_dem.h).
+//
+// Owns all device state of one sample, runs the t = 0 set-up
+// (md_calculationuniaxialcompression.cu:155-212), the loading loop in chunks
+// of graph-free launches with the controller on device (no per-step host
+// sync, unlike md_results.cu:58), the candidate-list rebuilds, and the
+// post-processing of the stress-strain curve (md_results.cu:147-187).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rockfit_dem.h"
+#include "rf_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932384626433832795;  // MC_pi
+thread_local std::string g_err;
+
+rf_status fail(rf_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(RF_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Per-bond strength: lognormal with the given mean and coefficient of
+// variation, keyed by the unordered pair so both directed copies agree.
+// Constant (the reference's single m_Ec / m_Gc) when cv == 0.
+double bond_strength(uint64_t seed, int64_t lo, int64_t hi, int stream, double mean, double cv) {
+    if (!(cv > 0.0)) return mean;
+    uint64_t base = seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(lo + 1)) ^
+                    (0xC2B2AE3D27D4EB4FULL * (uint64_t)(hi + 1)) ^
+                    (0x165667B19E3779F9ULL * (uint64_t)(stream + 1));
+    uint64_t h1 = mix64(base);
+    uint64_t h2 = mix64(h1 ^ 0xD6E8FEB86659FD93ULL);
+    double u1 = ((double)(h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    double u2 = ((double)(h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
+    double s2 = std::log(1.0 + cv * cv);
+    double s = std::sqrt(s2);
+    return mean * std::exp(s * z - 0.5 * s2);
+}
+
+struct Buf {
+    void* p = nullptr;
+};
+
+}  // namespace
+
+struct rf_ctx {
+    int64_t n = 0;
+    int32_t S = 32, C = 64, B = 32;
+    rf_material mat{};
+    rf_derived der{};
+    rf_box box{};
+    rf_options opt{};
+    rf::DevState d{};
+    rf::DevParams p{};
+    rf::Ctrl* ctrl = nullptr;
+    rf::Ctrl hctrl{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool built = false;
+    bool has_test = false;
+    rf_platen_cfg cfg{};
+    int64_t rebuilds = 0;
+    int64_t* dcount = nullptr;
+    std::vector<void*> allocs;
+    int32_t chunk = 256;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    double last_kernel_ms = 0.0, last_total_ms = 0.0;
+    int64_t last_steps = 0;
+    double ld = 0.0, destruction = 0.0;
+
+    template <class T>
+    rf_status alloc(T** out, size_t count) {
+        void* ptr = nullptr;
+        hipError_t e = hipMalloc(&ptr, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess)
+            return fail(RF_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                        hipGetErrorString(e));
+        hipMemset(ptr, 0, std::max<size_t>(count, 1) * sizeof(T));
+        allocs.push_back(ptr);
+        *out = (T*)ptr;
+        return RF_OK;
+    }
+};
+
+extern "C" {
+
+const char* rf_last_error(void) { return g_err.c_str(); }
+
+int32_t rf_version(void) { return RF_ABI_VERSION; }
+
+// potential_data::CalculateParameters (md_data_types.h:155-187) with the
+// smallest particle setting both time scales.
+rf_status rf_derive(const rf_material* m, double r_min, double r_max, rf_derived* d) {
+    if (!m || !d || !(r_min > 0.0) || r_max < r_min) return fail(RF_ERR_ARG, "rf_derive: bad args");
+    d->r_min = r_min;
+    d->r_max = r_max;
+    d->G_bond = 0.5 * m->E_bond / (1.0 + m->nu);
+    d->G_contact = 0.5 * m->E_contact / (1.0 + m->nu);
+    d->hm_E = 0.5 * m->E_contact / (1.0 - m->nu * m->nu);
+    d->hm_G = 0.5 * d->G_contact / (2.0 - m->nu);
+    const double le = std::log(m->restitution);
+    d->p_A = le / std::sqrt(kPi * kPi + le * le);
+    const double r = r_min;
+    d->T_rayleigh = kPi * r * std::sqrt(2.0 * m->density * (1.0 + m->nu)) /
+                    (std::sqrt(m->E_contact) * (0.163 * m->nu + 0.8766));
+    const double b = m->bond_radius_ratio > 0.0 ? m->bond_radius_ratio * r : m->bond_radius;
+    const double bS = kPi * b * b;
+    const double pV = 4.0 * (1.0 / 3.0) * kPi * r * r * r;
+    const double pm = m->density * pV;
+    const double t0 = m->E_bond * bS / (2.0 * r);
+    const double t1 = d->G_bond * bS / (2.0 * r);
+    d->T_bond = 2.0 * std::sqrt(pm / (t0 > t1 ? t0 : t1));
+    if (m->dt > 0.0)
+        d->dt = m->dt;
+    else
+        d->dt = (d->T_rayleigh < d->T_bond) ? 0.05 * d->T_rayleigh : 0.05 * d->T_bond;
+    return RF_OK;
+}
+
+// sample_data (md_data_types.h:87-124) and cell_data (:42-55) in FP64.
+rf_status rf_box_reference(const double space[3], double cell, rf_box* b) {
+    if (!space || !b || !(cell > 0.0)) return fail(RF_ERR_ARG, "rf_box_reference: bad args");
+    const double A = 1.01 * cell;
+    double B[3];
+    for (int k = 0; k < 3; ++k) {
+        B[k] = A + space[k];
+        b->center[k] = 0.5 * (A + B[k]);
+        b->half[k] = 0.5 * space[k];
+    }
+    b->hidden[0] = A + 0.5 * space[0];
+    b->hidden[1] = A + 0.5 * space[1];
+    b->hidden[2] = (std::floor(B[2] / cell) + 3 - 0.5) * cell;
+    b->cell = cell;
+    const double inv = 1.0 / cell;
+    b->dims[0] = (int32_t)std::floor(B[0] * inv) + 1;
+    b->dims[1] = (int32_t)std::floor(B[1] * inv) + 1;
+    b->dims[2] = (int32_t)std::floor(B[2] * inv) + 4;
+    b->pad_ = 0;
+    return RF_OK;
+}
+
+rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* box,
+                    const rf_options* opt, rf_ctx** out) {
+    if (!s || !mat || !box || !out || s->n <= 0 || !s->x || !s->y || !s->z || !s->r)
+        return fail(RF_ERR_ARG, "rf_create: null or empty input");
+    *out = nullptr;
+    rf_options o{};
+    o.slots = 32;
+    o.cand_cap = 64;
+    o.skin = -1.0;
+    o.device = 0;
+    o.block = rf::kBlock;
+    if (opt) {
+        if (opt->slots > 0) o.slots = opt->slots;
+        if (opt->cand_cap > 0) o.cand_cap = opt->cand_cap;
+        if (opt->skin >= 0.0) o.skin = opt->skin;
+        o.device = opt->device;
+    }
+    if (o.slots > rf::kMaxSlots)
+        return fail(RF_ERR_ARG, "slots=%d exceeds the %d lanes of a particle group", o.slots,
+                    rf::kMaxSlots);
+    if (o.cand_cap > rf::kMaxCand)
+        return fail(RF_ERR_ARG, "cand_cap=%d exceeds %d", o.cand_cap, rf::kMaxCand);
+    if (!(mat->acut > 0.0) || !(box->cell >= 2.0 * mat->acut * 0.999999999))
+        return fail(RF_ERR_ARG, "cell edge %.6g must be >= 2*acut (acut %.6g)", box->cell,
+                    mat->acut);
+    if (o.skin < 0.0) o.skin = 0.125 * mat->acut;
+    if (mat->acut + o.skin > box->cell)
+        return fail(RF_ERR_ARG, "acut + skin must not exceed the cell edge");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(RF_ERR_NODEVICE, "no HIP device available");
+    if (o.device < 0 || o.device >= ndev)
+        return fail(RF_ERR_ARG, "device %d out of range (%d devices)", o.device, ndev);
+    HIP_TRY(hipSetDevice(o.device));
+
+    rf_ctx* c = new rf_ctx();
+    c->n = s->n;
+    c->S = o.slots;
+    c->C = o.cand_cap;
+    c->B = rf::kMaxSlots;
+    c->mat = *mat;
+    c->box = *box;
+    c->opt = o;
+    c->device = o.device;
+    double rmin = s->r[0], rmax = s->r[0];
+    for (int64_t i = 0; i < s->n; ++i) {
+        if (!(s->r[i] > 0.0)) {
+            delete c;
+            return fail(RF_ERR_ARG, "radius of particle %lld is not positive", (long long)i);
+        }
+        rmin = std::min(rmin, s->r[i]);
+        rmax = std::max(rmax, s->r[i]);
+    }
+    rf_derive(mat, rmin, rmax, &c->der);
+    const int64_t n = c->n;
+    auto bail = [&](rf_status st) {
+        for (void* ptr : c->allocs) hipFree(ptr);
+        delete c;
+        return st;
+    };
+    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(RF_ERR_HIP, "hipStreamCreate"));
+    rf::DevState& d = c->d;
+    const int64_t nS = n * c->S, nC = n * c->C, nB = n * c->B;
+    rf_status st = RF_OK;
+#define ALLOC(ptr, count)                      \
+    if ((st = c->alloc(&(ptr), (count))) != RF_OK) return bail(st);
+    for (int b = 0; b < 2; ++b) {
+        ALLOC(d.pos[b], n);
+        ALLOC(d.vel[b], n);
+        ALLOC(d.angv[b], n);
+    }
+    ALLOC(d.pk, n);
+    ALLOC(d.F, n);
+    ALLOC(d.M, n);
+    ALLOC(d.rbuild, n);
+    ALLOC(d.nbr, nS);
+    ALLOC(d.otx, nS);
+    ALLOC(d.oty, nS);
+    ALLOC(d.otz, nS);
+    ALLOC(d.occ, n);
+    ALLOC(d.bnd, n);
+    ALLOC(d.otnz, n);
+    ALLOC(d.nbond, n);
+    ALLOC(d.bslot, nB);
+    ALLOC(d.bpart, nB);
+    ALLOC(d.binvL0, nB);
+    ALLOC(d.bsc, nB);
+    ALLOC(d.btc, nB);
+    ALLOC(d.bpnx, nB); ALLOC(d.bpny, nB); ALLOC(d.bpnz, nB);
+    ALLOC(d.botx, nB); ALLOC(d.boty, nB); ALLOC(d.botz, nB);
+    ALLOC(d.bmnx, nB); ALLOC(d.bmny, nB); ALLOC(d.bmnz, nB);
+    ALLOC(d.bmtx, nB); ALLOC(d.bmty, nB); ALLOC(d.bmtz, nB);
+    ALLOC(d.cand, nC);
+    ALLOC(d.cslot, nC);
+    ALLOC(d.ncand, n);
+    const int64_t ncell = (int64_t)box->dims[0] * box->dims[1] * box->dims[2];
+    ALLOC(d.ckey, n);
+    ALLOC(d.ccount, ncell);
+    ALLOC(d.cstart, ncell);
+    ALLOC(d.cfill, ncell);
+    ALLOC(d.ctmp, n);
+    ALLOC(d.sorted, n);
+    const int64_t nblk = (n + rf::kPerBlock - 1) / rf::kPerBlock;
+    ALLOC(d.part_b, nblk);
+    ALLOC(d.part_t, nblk);
+    ALLOC(d.part_bonds, nblk);
+    ALLOC(c->ctrl, 1);
+    ALLOC(c->dcount, 4);
+#undef ALLOC
+
+    // particle state: (x, y, z, r), (0, 0, 0, m), (0, 0, 0, 0); per-particle
+    // dt/m and dt/I as md_data_types.h:159-163, 185-186.
+    std::vector<double4> hp(n), hv(n), hw(n);
+    std::vector<double2> hk(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const double r = s->r[i];
+        const double pV = 4.0 * (1.0 / 3.0) * kPi * r * r * r;
+        const double pm = mat->density * pV;
+        const double pI = 0.4 * pm * r * r;
+        hp[i] = make_double4(s->x[i], s->y[i], s->z[i], r);
+        hv[i] = make_double4(0.0, 0.0, 0.0, pm);
+        hw[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+        hk[i] = make_double2((1.0 / pm) * c->der.dt, c->der.dt / pI);
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (hipMemcpy(d.pos[b], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice) ||
+            hipMemcpy(d.vel[b], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice) ||
+            hipMemcpy(d.angv[b], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice))
+            return bail(fail(RF_ERR_HIP, "upload of particle state failed"));
+    }
+    if (hipMemcpy(d.pk, hk.data(), n * sizeof(double2), hipMemcpyHostToDevice))
+        return bail(fail(RF_ERR_HIP, "upload of particle constants failed"));
+
+    rf::DevParams& p = c->p;
+    p.n = n;
+    p.S = c->S;
+    p.C = c->C;
+    p.B = c->B;
+    p.nblk = (int32_t)nblk;
+    p.dt = c->der.dt;
+    p.E = mat->E_bond;
+    p.G = c->der.G_bond;
+    p.hmE = c->der.hm_E;
+    p.hmG = c->der.hm_G;
+    p.pA = c->der.p_A;
+    p.mu = mat->friction;
+    p.mur = mat->rolling_friction;
+    p.acut2 = mat->acut * mat->acut;
+    p.cell = box->cell;
+    p.inv_cell = 1.0 / box->cell;
+    p.cand_r2 = (mat->acut + o.skin) * (mat->acut + o.skin);
+    p.skin_half2 = 0.25 * o.skin * o.skin;
+    p.b_ratio = mat->bond_radius_ratio;
+    p.b_abs = mat->bond_radius;
+    p.bond_gap = mat->bond_gap;
+    p.quirks = mat->quirks;
+    p.nx = box->dims[0];
+    p.ny = box->dims[1];
+    p.nz = box->dims[2];
+    p.ncell = ncell;
+    for (int k = 0; k < 3; ++k) {
+        p.cen[k] = box->center[k];
+        p.half[k] = box->half[k];
+        p.hid[k] = box->hidden[k];
+    }
+    p.kind = RF_TEST_NONE;
+    p.axis = 2;
+    p.dnsave = 1;
+    p.store_fm = 1;
+    p.curve_cap = 0;
+    std::memset(&c->hctrl, 0, sizeof(c->hctrl));
+    if (hipMemcpy(c->ctrl, &c->hctrl, sizeof(rf::Ctrl), hipMemcpyHostToDevice))
+        return bail(fail(RF_ERR_HIP, "upload of control block failed"));
+    *out = c;
+    return RF_OK;
+}
+
+rf_status rf_destroy(rf_ctx* c) {
+    if (!c) return RF_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev) hipEventDestroy(e);
+    for (void* ptr : c->allocs) hipFree(ptr);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return RF_OK;
+}
+
+int64_t rf_num_particles(const rf_ctx* c) { return c ? c->n : 0; }
+int32_t rf_num_slots(const rf_ctx* c) { return c ? c->S : 0; }
+
+}  // extern "C"
+
+namespace {
+
+rf_status sync_ctrl(rf_ctx* c) {
+    HIP_TRY(hipMemcpyAsync(&c->hctrl, c->ctrl, sizeof(rf::Ctrl), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return RF_OK;
+}
+
+rf_status device_error(rf_ctx* c) {
+    const int e = c->hctrl.err;
+    if (e & rf::kErrGrid) return fail(RF_ERR_GRID, "a particle left the padded cell grid");
+    if (e & rf::kErrRowOverflow)
+        return fail(RF_ERR_OVERFLOW, "interaction row overflow (more than %d partners)", c->S);
+    if (e & rf::kErrCandOverflow)
+        return fail(RF_ERR_OVERFLOW, "candidate list overflow (more than %d within acut+skin)",
+                    c->C);
+    return fail(RF_ERR_STATE, "device error flags 0x%x", e);
+}
+
+// Cell sort of the current positions and fresh candidate lists.
+rf_status rebuild(rf_ctx* c) {
+    HIP_TRY(rf::launch_cell_keys(c->d, c->p, c->ctrl, c->stream));
+    HIP_TRY(rf::launch_cell_sort(c->d, c->p, c->ctrl, c->stream));
+    HIP_TRY(rf::launch_build_candidates(c->d, c->p, c->ctrl, c->stream));
+    HIP_TRY(rf::launch_clear_flags(c->ctrl, c->stream));
+    c->rebuilds++;
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_build_bonds(rf_ctx* c) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    rf::DevState& d = c->d;
+    HIP_TRY(rf::launch_cell_keys(d, c->p, c->ctrl, c->stream));
+    HIP_TRY(rf::launch_cell_sort(d, c->p, c->ctrl, c->stream));
+    HIP_TRY(rf::launch_construct_rows(d, c->p, c->ctrl, c->stream));
+    HIP_TRY(rf::launch_construct_bonds(d, c->p, c->ctrl, c->stream));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    if (c->hctrl.err) return device_error(c);
+    // per-bond strengths on the host (same draw as every other host tool)
+    const int64_t n = c->n, B = c->B;
+    std::vector<int32_t> nb(n), part(n * B);
+    HIP_TRY(hipMemcpy(nb.data(), d.nbond, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(part.data(), d.bpart, n * B * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> sc(n * B, 0.0), tc(n * B, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int b = 0; b < nb[i]; ++b) {
+            const int64_t j = part[i * B + b];
+            const int64_t lo = std::min(i, j), hi = std::max(i, j);
+            sc[i * B + b] = bond_strength(c->mat.strength_seed, lo, hi, 0, c->mat.sigma_c,
+                                          c->mat.sigma_c_cv);
+            tc[i * B + b] = bond_strength(c->mat.strength_seed, lo, hi, 1, c->mat.tau_c,
+                                          c->mat.tau_c_cv);
+        }
+    HIP_TRY(hipMemcpy(d.bsc, sc.data(), n * B * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.btc, tc.data(), n * B * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(rf::launch_prune_initial(d, c->p, c->ctrl, c->stream));
+    st = rebuild(c);
+    if (st) return st;
+    st = sync_ctrl(c);
+    if (st) return st;
+    if (c->hctrl.err) return device_error(c);
+    c->built = true;
+    return RF_OK;
+}
+
+// compression_data::CalculateUParameters / CalculateBParameters
+// (md_data_types.h:212-242, 380-409) and ResultsUInit (md_results.cu:16-29).
+rf_status rf_set_test(rf_ctx* c, const rf_platen_cfg* cfg) {
+    if (!c || !cfg) return fail(RF_ERR_ARG, "null argument");
+    if (cfg->kind < RF_TEST_NONE || cfg->kind > RF_TEST_BRAZILIAN)
+        return fail(RF_ERR_ARG, "unknown test kind %d", cfg->kind);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->cfg = *cfg;
+    c->has_test = true;
+    rf::DevParams& p = c->p;
+    rf::Ctrl& h = c->hctrl;
+    const rf::Ctrl old = h;
+    std::memset(&h, 0, sizeof(h));
+    h.cur = old.cur;
+    p.kind = cfg->kind;
+    p.dnsave = cfg->dnsave > 0 ? cfg->dnsave : 1;
+    p.max_steps = cfg->max_steps;
+    if (cfg->kind != RF_TEST_NONE) {
+        const int ax = cfg->kind == RF_TEST_BRAZILIAN ? 1 : 2;
+        p.axis = ax;
+        const double sz = cfg->size[ax];
+        h.size_axis = sz;
+        h.size0 = sz;
+        h.bottom = cfg->center[ax] - 0.5 * sz;
+        h.top = cfg->center[ax] + 0.5 * sz;
+        for (int k = 0; k < 3; ++k) p.scope_c[k] = cfg->center[k];
+        const double hx = 0.5 * cfg->size[0];
+        double area;
+        if (cfg->kind == RF_TEST_BRAZILIAN) {
+            const double hz = 0.5 * cfg->size[2];
+            const double Rs = (hx > hz) ? hx : 1.05 * hz;
+            p.rr_scope = Rs * Rs;
+            area = 0.5 * 2 * kPi * hx * cfg->size[2];
+            p.zero_stress = 0.5 * cfg->min_stress;
+        } else {
+            const double Rs = 1.5 * hx;
+            p.rr_scope = Rs * Rs;
+            area = kPi * (hx + cfg->r_ref) * (hx + cfg->r_ref);
+            p.zero_stress = 0.33 * cfg->min_stress;
+        }
+        p.inv_area = 1.0 / area;
+        p.zcut = c->box.center[2] + c->box.half[2];
+        p.Ew = cfg->wall_modulus_factor * c->der.hm_E;
+        p.Gw = cfg->wall_modulus_factor * c->der.hm_G;
+        p.v_load = cfg->v_load;
+        p.min_stress = cfg->min_stress;
+        p.max_strain = cfg->max_strain;
+        p.cond_fracture = cfg->cond_fracture;
+        int64_t cap = cfg->curve_cap;
+        if (cap <= 0) {
+            const int64_t dn = p.dnsave;
+            const double est = 1.5 * cfg->max_strain * sz / (std::fabs(cfg->v_load) * c->der.dt * dn);
+            cap = (int64_t)est + 16;
+            if (cfg->max_steps > 0 && cfg->max_steps / dn + 16 < cap) cap = cfg->max_steps / dn + 16;
+        }
+        p.curve_cap = cap;
+        if (c->d.curve) {
+            hipFree(c->d.curve);
+            hipFree(c->d.bcount);
+            c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)c->d.curve),
+                            c->allocs.end());
+            c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)c->d.bcount),
+                            c->allocs.end());
+            c->d.curve = nullptr;
+            c->d.bcount = nullptr;
+        }
+        rf_status st = c->alloc(&c->d.curve, 3 * (cap + 2));
+        if (st) return st;
+        st = c->alloc(&c->d.bcount, cap + 2);
+        if (st) return st;
+        if (!(cfg->max_strain > 0.0)) h.done = 1;
+    } else {
+        p.curve_cap = 0;
+    }
+    h.halt = h.done ? 1 : 0;
+    HIP_TRY(hipMemcpy(c->ctrl, &h, sizeof(rf::Ctrl), hipMemcpyHostToDevice));
+    c->ld = 0.0;
+    c->destruction = 0.0;
+    return RF_OK;
+}
+
+rf_status rf_set_timing(rf_ctx* c, int32_t enable) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    c->timing = enable != 0;
+    return RF_OK;
+}
+
+rf_status rf_set_chunk(rf_ctx* c, int32_t steps) {
+    if (!c || steps <= 0) return fail(RF_ERR_ARG, "bad chunk");
+    c->chunk = steps;
+    return RF_OK;
+}
+
+rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
+    if (done_steps) *done_steps = 0;
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    if (!c->built) return fail(RF_ERR_STATE, "rf_step before rf_build_bonds");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    const int64_t start = c->hctrl.step;
+    int64_t remaining = nsteps;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, c->stream));
+    double kernel_ms = 0.0;
+    while (remaining > 0 && !c->hctrl.done) {
+        const int64_t k = std::min<int64_t>(remaining, c->chunk);
+        if (c->timing && (int64_t)c->ev.size() < 2 * k) {
+            while ((int64_t)c->ev.size() < 2 * k) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreate(&e));
+                c->ev.push_back(e);
+            }
+        }
+        const int64_t before = c->hctrl.step;
+        for (int64_t s = 0; s < k; ++s) {
+            if (c->timing) HIP_TRY(hipEventRecord(c->ev[2 * s], c->stream));
+            HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
+            if (c->timing) HIP_TRY(hipEventRecord(c->ev[2 * s + 1], c->stream));
+            HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+        }
+        st = sync_ctrl(c);
+        if (st) return st;
+        const int64_t executed = c->hctrl.step - before;
+        if (c->timing)
+            for (int64_t s = 0; s < executed; ++s) {
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
+                kernel_ms += ms;
+            }
+        remaining -= executed;
+        if (c->hctrl.err) {
+            hipEventDestroy(t0);
+            hipEventDestroy(t1);
+            return device_error(c);
+        }
+        if (c->hctrl.rebuild) {
+            st = rebuild(c);
+            if (st) return st;
+            st = sync_ctrl(c);
+            if (st) return st;
+            if (c->hctrl.err) return device_error(c);
+        } else if (executed == 0 && !c->hctrl.done) {
+            return fail(RF_ERR_STATE, "loading loop halted without a reason");
+        }
+    }
+    HIP_TRY(hipEventRecord(t1, c->stream));
+    HIP_TRY(hipEventSynchronize(t1));
+    float total = 0.f;
+    HIP_TRY(hipEventElapsedTime(&total, t0, t1));
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    c->last_total_ms = total;
+    c->last_kernel_ms = kernel_ms;
+    c->last_steps = c->hctrl.step - start;
+    if (done_steps) *done_steps = c->last_steps;
+    return RF_OK;
+}
+
+rf_status rf_last_timing(rf_ctx* c, double* kms, double* tms, int64_t* steps) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    if (kms) *kms = c->last_kernel_ms;
+    if (tms) *tms = c->last_total_ms;
+    if (steps) *steps = c->last_steps;
+    return RF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// CalculateLinearityDeviation and CalculateDestruction (md_results.cu:147-187)
+// on the downloaded curve; zero-filled past the last sample.
+rf_status post_process(rf_ctx* c) {
+    const int64_t cap = c->p.curve_cap;
+    if (cap <= 0) return RF_OK;
+    std::vector<double> h(3 * (cap + 2));
+    std::vector<int64_t> bc(cap + 2);
+    HIP_TRY(hipMemcpy(h.data(), c->d.curve, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bc.data(), c->d.bcount, bc.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const rf::Ctrl& t = c->hctrl;
+    if (c->cfg.kind == RF_TEST_UNIAXIAL) {
+        double l_eps = 0, l_deps = 0, l_A = t.peak_stress / t.peak_strain, l_dS = 0, ld = 0;
+        int64_t l_i = 0;
+        do {
+            ld += l_deps * l_dS * l_dS;
+            ++l_i;
+            if (l_i + 1 > cap + 1) break;
+            l_eps = -h[3 * l_i];
+            l_deps = -0.5 * (h[3 * (l_i + 1)] - h[3 * (l_i - 1)]);
+            l_dS = (l_A * l_eps - 0.5 * (h[3 * l_i + 2] - h[3 * l_i + 1]));
+        } while (l_eps < t.peak_strain - 2 * l_deps && l_i < t.stepsave);
+        ld /= (t.peak_strain * t.peak_stress * t.peak_stress);
+        ld += 1.0;
+        c->ld = ld;
+    }
+    if (t.stepsave >= 1 && bc[0] > 0) c->destruction = (double)bc[t.stepsave - 1] / (double)bc[0];
+    return RF_OK;
+}
+
+void fill_state(const rf_ctx* c, rf_test_state* o) {
+    const rf::Ctrl& t = c->hctrl;
+    std::memset(o, 0, sizeof(*o));
+    o->top = t.top; o->bottom = t.bottom; o->v = t.vw; o->size_axis = t.size_axis;
+    o->size0 = t.size0; o->strain = t.strain; o->stress = t.stress;
+    o->peak_stress = t.peak_stress; o->peak_strain = t.peak_strain;
+    o->sfl0 = t.sfl0; o->sfl1 = t.sfl1;
+    o->linearity_deviation = c->ld; o->destruction = c->destruction;
+    o->step = t.step; o->steps_stop = t.steps_stop; o->n_curve = t.stepsave;
+    o->loading = t.loading; o->fracture = t.fracture; o->done = t.done;
+    o->kind = c->has_test ? c->cfg.kind : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_run(rf_ctx* c, rf_test_state* out) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    if (!c->has_test || c->cfg.kind == RF_TEST_NONE)
+        return fail(RF_ERR_STATE, "rf_run needs a platen test (rf_set_test)");
+    int64_t done = 0;
+    do {
+        rf_status st = rf_step(c, INT64_MAX / 4, &done);
+        if (st) return st;
+    } while (!c->hctrl.done && done > 0);
+    rf_status st = post_process(c);
+    if (st) return st;
+    if (out) fill_state(c, out);
+    return RF_OK;
+}
+
+rf_status rf_get_test_state(rf_ctx* c, rf_test_state* out) {
+    if (!c || !out) return fail(RF_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    if (c->hctrl.done) {
+        st = post_process(c);
+        if (st) return st;
+    }
+    fill_state(c, out);
+    return RF_OK;
+}
+
+rf_status rf_get_curve(rf_ctx* c, double* out, int64_t cap, int64_t* n) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    int64_t m = std::min<int64_t>(c->hctrl.stepsave, c->p.curve_cap);
+    if (m > cap) m = cap;
+    if (out && m > 0)
+        HIP_TRY(hipMemcpy(out, c->d.curve, 3 * m * sizeof(double), hipMemcpyDeviceToHost));
+    if (n) *n = m;
+    return RF_OK;
+}
+
+rf_status rf_get_bond_counts(rf_ctx* c, int64_t* out, int64_t cap, int64_t* n) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    int64_t m = std::min<int64_t>(c->hctrl.stepsave, c->p.curve_cap);
+    if (m > cap) m = cap;
+    if (out && m > 0)
+        HIP_TRY(hipMemcpy(out, c->d.bcount, m * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (n) *n = m;
+    return RF_OK;
+}
+
+rf_status rf_get_counters(rf_ctx* c, rf_counters* out) {
+    if (!c || !out) return fail(RF_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipMemsetAsync(c->dcount, 0, 4 * sizeof(int64_t), c->stream));
+    HIP_TRY(rf::launch_counters(c->d, c->p, c->ctrl, c->dcount, c->stream));
+    int64_t h[4];
+    HIP_TRY(hipMemcpyAsync(h, c->dcount, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    std::memset(out, 0, sizeof(*out));
+    out->n = c->n;
+    out->occupied = h[0];
+    out->bonds = h[1];
+    out->contacts = h[2];
+    out->candidates = h[3];
+    out->steps = c->hctrl.step;
+    out->rebuilds = c->rebuilds;
+    return RF_OK;
+}
+
+rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
+    if (!c || !v) return fail(RF_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    const int64_t n = c->n, S = c->S, B = c->B;
+    const rf::DevState& d = c->d;
+    const int cur = c->hctrl.cur;
+    std::vector<double4> hp(n), hv(n), hw(n), hf(n), hm(n);
+    HIP_TRY(hipMemcpy(hp.data(), d.pos[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hv.data(), d.vel[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hw.data(), d.angv[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hf.data(), d.F, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hm.data(), d.M, n * sizeof(double4), hipMemcpyDeviceToHost));
+    auto split = [n](double* dst, const std::vector<double4>& src) {
+        if (!dst) return;
+        for (int64_t i = 0; i < n; ++i) {
+            dst[i] = src[i].x;
+            dst[n + i] = src[i].y;
+            dst[2 * n + i] = src[i].z;
+        }
+    };
+    split(v->R, hp); split(v->V, hv); split(v->W, hw); split(v->F, hf); split(v->M, hm);
+    std::vector<int32_t> nbr(n * S), nbond(n), bpart(n * B);
+    std::vector<uint32_t> occ(n), bnd(n), otnz(n);
+    std::vector<double> otx(n * S), oty(n * S), otz(n * S);
+    std::vector<int8_t> bslot(n * B);
+    HIP_TRY(hipMemcpy(nbr.data(), d.nbr, n * S * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(occ.data(), d.occ, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bnd.data(), d.bnd, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(otnz.data(), d.otnz, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(otx.data(), d.otx, n * S * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(oty.data(), d.oty, n * S * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(otz.data(), d.otz, n * S * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(nbond.data(), d.nbond, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bslot.data(), d.bslot, n * B, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bpart.data(), d.bpart, n * B * 4, hipMemcpyDeviceToHost));
+    std::vector<double> bf[15];
+    double* bsrc[15] = {d.binvL0, d.bsc, d.btc, d.bpnx, d.bpny, d.bpnz, d.botx, d.boty, d.botz,
+                        d.bmnx, d.bmny, d.bmnz, d.bmtx, d.bmty, d.bmtz};
+    for (int f = 0; f < 15; ++f) {
+        bf[f].resize(n * B);
+        HIP_TRY(hipMemcpy(bf[f].data(), bsrc[f], n * B * 8, hipMemcpyDeviceToHost));
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t k = 0; k < S; ++k) {
+            const int64_t q = i * S + k;
+            const bool oc = (occ[i] >> k) & 1u;
+            const bool bd = oc && ((bnd[i] >> k) & 1u);
+            if (v->nbr) v->nbr[q] = oc ? nbr[q] : -1;
+            if (v->type) v->type[q] = bd ? 1 : 0;
+            if (v->bidx) v->bidx[q] = -1;
+            const bool onz = oc && !bd && ((otnz[i] >> k) & 1u);
+            if (v->o_t) {
+                v->o_t[3 * q] = onz ? otx[q] : 0.0;
+                v->o_t[3 * q + 1] = onz ? oty[q] : 0.0;
+                v->o_t[3 * q + 2] = onz ? otz[q] : 0.0;
+            }
+            if (v->inv_L0) v->inv_L0[q] = 0.0;
+            if (v->sigma_c) v->sigma_c[q] = 0.0;
+            if (v->tau_c) v->tau_c[q] = 0.0;
+            for (int cc = 0; cc < 3; ++cc) {
+                if (v->prev_n) v->prev_n[3 * q + cc] = 0.0;
+                if (v->m_n) v->m_n[3 * q + cc] = 0.0;
+                if (v->m_t) v->m_t[3 * q + cc] = 0.0;
+            }
+        }
+        for (int b = 0; b < nbond[i]; ++b) {
+            const int64_t qb = i * B + b;
+            const int ks = bslot[qb];
+            if (ks < 0 || !((bnd[i] >> ks) & 1u) || !((occ[i] >> ks) & 1u)) continue;
+            const int64_t q = i * S + ks;
+            if (v->bidx) v->bidx[q] = (int8_t)b;
+            if (v->inv_L0) v->inv_L0[q] = bf[0][qb];
+            if (v->sigma_c) v->sigma_c[q] = bf[1][qb];
+            if (v->tau_c) v->tau_c[q] = bf[2][qb];
+            for (int cc = 0; cc < 3; ++cc) {
+                if (v->prev_n) v->prev_n[3 * q + cc] = bf[3 + cc][qb];
+                if (v->o_t) v->o_t[3 * q + cc] = bf[6 + cc][qb];
+                if (v->m_n) v->m_n[3 * q + cc] = bf[9 + cc][qb];
+                if (v->m_t) v->m_t[3 * q + cc] = bf[12 + cc][qb];
+            }
+        }
+    }
+    return RF_OK;
+}
+
+rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
+    if (!c || !v) return fail(RF_ERR_ARG, "null argument");
+    if (!v->R || !v->V || !v->W || !v->nbr || !v->type || !v->bidx)
+        return fail(RF_ERR_ARG, "rf_set_state needs R, V, W, nbr, type and bidx");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    const int64_t n = c->n, S = c->S, B = c->B;
+    rf::DevState& d = c->d;
+    const int cur = c->hctrl.cur;
+    std::vector<double4> hp(n), hv(n), hw(n), hf(n), hm(n);
+    HIP_TRY(hipMemcpy(hp.data(), d.pos[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hv.data(), d.vel[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+        hp[i] = make_double4(v->R[i], v->R[n + i], v->R[2 * n + i], hp[i].w);
+        hv[i] = make_double4(v->V[i], v->V[n + i], v->V[2 * n + i], hv[i].w);
+        hw[i] = make_double4(v->W[i], v->W[n + i], v->W[2 * n + i], 0.0);
+        hf[i] = v->F ? make_double4(v->F[i], v->F[n + i], v->F[2 * n + i], 0.0)
+                     : make_double4(0.0, 0.0, 0.0, 0.0);
+        hm[i] = v->M ? make_double4(v->M[i], v->M[n + i], v->M[2 * n + i], 0.0)
+                     : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    HIP_TRY(hipMemcpy(d.pos[cur], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.vel[cur], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.angv[cur], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.F, hf.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.M, hm.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    std::vector<int32_t> nbr(n * S, 0), nbond(n, 0), bpart(n * B, 0);
+    std::vector<uint32_t> occ(n, 0), bnd(n, 0), otnz(n, 0);
+    std::vector<double> otx(n * S, 0.0), oty(n * S, 0.0), otz(n * S, 0.0);
+    std::vector<int8_t> bslot(n * B, -1);
+    std::vector<double> bf[15];
+    for (int f = 0; f < 15; ++f) bf[f].assign(n * B, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = 0; k < S; ++k) {
+            const int64_t q = i * S + k;
+            if (v->nbr[q] < 0) continue;
+            if (v->nbr[q] >= n) return fail(RF_ERR_ARG, "slot partner out of range");
+            occ[i] |= 1u << k;
+            nbr[q] = v->nbr[q];
+            if (v->type[q] == 1) {
+                const int b = v->bidx[q];
+                if (b < 0 || b >= B) return fail(RF_ERR_ARG, "bond slot without a valid bidx");
+                bnd[i] |= 1u << k;
+                const int64_t qb = i * B + b;
+                bslot[qb] = (int8_t)k;
+                bpart[qb] = v->nbr[q];
+                nbond[i] = std::max(nbond[i], b + 1);
+                bf[0][qb] = v->inv_L0 ? v->inv_L0[q] : 0.0;
+                bf[1][qb] = v->sigma_c ? v->sigma_c[q] : 0.0;
+                bf[2][qb] = v->tau_c ? v->tau_c[q] : 0.0;
+                for (int cc = 0; cc < 3; ++cc) {
+                    bf[3 + cc][qb] = v->prev_n ? v->prev_n[3 * q + cc] : 0.0;
+                    bf[6 + cc][qb] = v->o_t ? v->o_t[3 * q + cc] : 0.0;
+                    bf[9 + cc][qb] = v->m_n ? v->m_n[3 * q + cc] : 0.0;
+                    bf[12 + cc][qb] = v->m_t ? v->m_t[3 * q + cc] : 0.0;
+                }
+            } else if (v->o_t) {
+                otx[q] = v->o_t[3 * q];
+                oty[q] = v->o_t[3 * q + 1];
+                otz[q] = v->o_t[3 * q + 2];
+                otnz[i] |= 1u << k;
+            }
+        }
+    HIP_TRY(hipMemcpy(d.nbr, nbr.data(), n * S * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.occ, occ.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.bnd, bnd.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.otnz, otnz.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.otx, otx.data(), n * S * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.oty, oty.data(), n * S * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.otz, otz.data(), n * S * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.nbond, nbond.data(), n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.bslot, bslot.data(), n * B, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.bpart, bpart.data(), n * B * 4, hipMemcpyHostToDevice));
+    double* bdst[15] = {d.binvL0, d.bsc, d.btc, d.bpnx, d.bpny, d.bpnz, d.botx, d.boty, d.botz,
+                        d.bmnx, d.bmny, d.bmnz, d.bmtx, d.bmty, d.bmtz};
+    for (int f = 0; f < 15; ++f)
+        HIP_TRY(hipMemcpy(bdst[f], bf[f].data(), n * B * 8, hipMemcpyHostToDevice));
+    st = rebuild(c);
+    if (st) return st;
+    st = sync_ctrl(c);
+    if (st) return st;
+    if (c->hctrl.err) return device_error(c);
+    c->built = true;
+    return RF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""Python handle on one sample in device memory (the C-ABI's rf_ctx).
+
+Mirrors the reference's test drivers: create the context from a sample,
+build rows and bonds (the t = 0 part of h_CalculationUniaxialCompression,
+md_calculationuniaxialcompression.cu:155-212), select a platen test and
+step or run it.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import check, lib
+
+STATE_FIELDS = [("R", 3, np.float64, False), ("V", 3, np.float64, False),
+                ("W", 3, np.float64, False), ("F", 3, np.float64, False),
+                ("M", 3, np.float64, False), ("nbr", 1, np.int32, True),
+                ("type", 1, np.uint8, True), ("inv_L0", 1, np.float64, True),
+                ("prev_n", 3, np.float64, True), ("o_t", 3, np.float64, True),
+                ("m_n", 3, np.float64, True), ("m_t", 3, np.float64, True),
+                ("sigma_c", 1, np.float64, True), ("tau_c", 1, np.float64, True),
+                ("bidx", 1, np.int8, True)]
+
+_CT = {np.float64: C.c_double, np.int32: C.c_int32, np.uint8: C.c_uint8, np.int8: C.c_int8}
+
+
+def alloc_state(n, slots):
+    st = {}
+    for name, comps, dt, per_slot in STATE_FIELDS:
+        st[name] = np.zeros(n * slots * comps if per_slot else 3 * n, dtype=dt)
+    return st
+
+
+def state_view(st):
+    v = abi.StateView()
+    for name, _, dt, _ in STATE_FIELDS:
+        a = st.get(name)
+        if a is not None:
+            setattr(v, name, a.ctypes.data_as(C.POINTER(_CT[dt])))
+    return v
+
+
+def derive(mat, r_min, r_max):
+    d = abi.Derived()
+    check(lib().rf_derive(C.byref(mat), float(r_min), float(r_max), C.byref(d)), "rf_derive")
+    return d
+
+
+def box_reference(space, cell):
+    b = abi.Box()
+    sp = (C.c_double * 3)(*[float(v) for v in space])
+    check(lib().rf_box_reference(sp, float(cell), C.byref(b)), "rf_box_reference")
+    return b
+
+
+class Context:
+    """One sample on the GPU."""
+
+    def __init__(self, xyz, r, mat, box, slots=32, cand_cap=64, skin=-1.0, device=0):
+        self.n = int(len(r))
+        self.slots = int(slots)
+        self._x = np.ascontiguousarray(xyz[:, 0], dtype=np.float64)
+        self._y = np.ascontiguousarray(xyz[:, 1], dtype=np.float64)
+        self._z = np.ascontiguousarray(xyz[:, 2], dtype=np.float64)
+        self._r = np.ascontiguousarray(r, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        s = abi.Sample(self.n, self._x.ctypes.data_as(dp), self._y.ctypes.data_as(dp),
+                       self._z.ctypes.data_as(dp), self._r.ctypes.data_as(dp))
+        opt = abi.Options(slots, cand_cap, skin, device, 256)
+        h = C.c_void_p()
+        self.h = None
+        check(lib().rf_create(C.byref(s), C.byref(mat), C.byref(box), C.byref(opt), C.byref(h)),
+              "rf_create")
+        self.h = h
+        self.mat = mat
+        self.box = box
+
+    def close(self):
+        if self.h:
+            lib().rf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_bonds(self):
+        check(lib().rf_build_bonds(self.h), "rf_build_bonds")
+
+    def set_test(self, cfg):
+        check(lib().rf_set_test(self.h, C.byref(cfg)), "rf_set_test")
+
+    def step(self, nsteps):
+        done = C.c_int64()
+        check(lib().rf_step(self.h, int(nsteps), C.byref(done)), "rf_step")
+        return done.value
+
+    def run(self):
+        t = abi.TestState()
+        check(lib().rf_run(self.h, C.byref(t)), "rf_run")
+        return t
+
+    def test_state(self):
+        t = abi.TestState()
+        check(lib().rf_get_test_state(self.h, C.byref(t)), "rf_get_test_state")
+        return t
+
+    def curve(self):
+        n = C.c_int64()
+        check(lib().rf_get_curve(self.h, None, 0, C.byref(n)), "rf_get_curve")
+        cap = 1 << 24
+        out = np.zeros((min(cap, max(1, self.test_state().n_curve)), 3))
+        check(lib().rf_get_curve(self.h, out.ctypes.data_as(C.c_void_p), len(out), C.byref(n)),
+              "rf_get_curve")
+        return out[:n.value].copy()
+
+    def bond_counts(self):
+        m = max(1, self.test_state().n_curve)
+        out = np.zeros(m, dtype=np.int64)
+        n = C.c_int64()
+        check(lib().rf_get_bond_counts(self.h, out.ctypes.data_as(C.c_void_p), m, C.byref(n)),
+              "rf_get_bond_counts")
+        return out[:n.value].copy()
+
+    def counters(self):
+        c = abi.Counters()
+        check(lib().rf_get_counters(self.h, C.byref(c)), "rf_get_counters")
+        return c
+
+    def get_state(self):
+        st = alloc_state(self.n, self.slots)
+        v = state_view(st)
+        check(lib().rf_get_state(self.h, C.byref(v)), "rf_get_state")
+        return st
+
+    def set_state(self, st):
+        v = state_view(st)
+        check(lib().rf_set_state(self.h, C.byref(v)), "rf_set_state")
+
+    def set_timing(self, on=True):
+        check(lib().rf_set_timing(self.h, 1 if on else 0), "rf_set_timing")
+
+    def set_chunk(self, steps):
+        check(lib().rf_set_chunk(self.h, int(steps)), "rf_set_chunk")
+
+    def last_timing(self):
+        k, t, s = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().rf_last_timing(self.h, C.byref(k), C.byref(t), C.byref(s)), "rf_last_timing")
+        return k.value, t.value, s.value

@@ -1,0 +1,240 @@
+/*
+ * rockfit_dem.h -- C-ABI of the MI355X (gfx950) bonded-particle DEM step.
+ *
+ * Plain C, plain pointers and sizes.  Every function returns an rf_status;
+ * rf_last_error() gives the text of the last failure on the calling thread.
+ *
+ * Units: every length, mass, time, force and stress that crosses this
+ * boundary is in the reference's reduced units (mm, mg, us; force 1e3 N,
+ * stress 1e9 Pa, density 1e3 kg/m^3, velocity 1e3 m/s), the units the
+ * reference uses on its device (md_phys_constants.h:5-14).
+ *
+ * Reference entry points this library replaces (file:line in
+ * MSMS-LAB/DEM-Rock-Calibration):
+ *   h_CalculationUniaxialCompression(md_task_data&, float*)   mp.h:16, md_calculationuniaxialcompression.cu:126
+ *   h_CalculationBrazilTest(md_task_data&, float*)            mp.h:17, md_calculationbraziltest.cu:116
+ *   h_LoadCylinderSample(mp_mdparameters_data&, uint, char[]) mp.h:22, md_createcylindersample.cu:441
+ * and, below them, the device kernels of the loading loop (md.h:15-216).
+ */
+#ifndef ROCKFIT_DEM_H
+#define ROCKFIT_DEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RF_ABI_VERSION 1
+
+typedef enum rf_status {
+    RF_OK = 0,
+    RF_ERR_ARG = 1,       /* invalid argument or inconsistent sizes */
+    RF_ERR_HIP = 2,       /* a HIP runtime call failed */
+    RF_ERR_OVERFLOW = 3,  /* an interaction row (or candidate list) is full */
+    RF_ERR_STATE = 4,     /* call out of order (e.g. step before build) */
+    RF_ERR_GRID = 5,      /* a particle lies outside the padded cell grid */
+    RF_ERR_ALLOC = 6,     /* device or host allocation failed */
+    RF_ERR_IO = 7,        /* file could not be read or written */
+    RF_ERR_NODEVICE = 8   /* no usable GPU */
+} rf_status;
+
+/* Reference quirks reproduced by default (bitmask, rf_material.quirks). */
+#define RF_QUIRK_BEND_DOUBLE_SCALE  0x1u  /* bending moment scaled twice by dt*E/(4L0), md_calculateforcesdem3.cu:142-154 */
+#define RF_QUIRK_ROLL_RESET         0x2u  /* |W_i|^2<=1e-18 clears the whole contact moment sum, md_calculateforcesdem3.cu:333-338 */
+#define RF_QUIRK_SLOT0_SENTINEL     0x4u  /* particle 0 free-slot search uses 0 as "none", md_interaction_list_reconstructbpm.cu:28-33 */
+#define RF_QUIRKS_REFERENCE (RF_QUIRK_BEND_DOUBLE_SCALE | RF_QUIRK_ROLL_RESET | RF_QUIRK_SLOT0_SENTINEL)
+
+/* Material and contact-law parameters (potential_data, md_data_types.h:148-188,
+ * values set in SetInitialParameters, mp_determineconjgradient.cu:15-47).
+ * The reference has one radius and one modulus; here radii are per particle
+ * and the contact modulus is separate from the bond modulus.  With equal
+ * radii and E_contact == E_bond the formulas reduce to the reference ones. */
+typedef struct rf_material {
+    double E_bond;            /* bond Young's modulus (m_E)                      */
+    double nu;                /* Poisson ratio (m_nu)                             */
+    double E_contact;         /* Hertz-Mindlin Young's modulus (reference: m_E)  */
+    double density;           /* m_ro                                            */
+    double restitution;       /* p_e                                             */
+    double friction;          /* p_mu                                            */
+    double rolling_friction;  /* p_mur                                           */
+    double bond_radius;       /* absolute bond radius b_r; used if ratio <= 0    */
+    double bond_radius_ratio; /* b = ratio * min(r_i, r_j) when > 0              */
+    double bond_gap;          /* bond if rho^2 < (r_i + r_j + gap)^2 (b_d)       */
+    double sigma_c;           /* normal bond strength (m_Ec), mean if cv > 0     */
+    double tau_c;             /* shear bond strength (m_Gc), mean if cv > 0      */
+    double sigma_c_cv;        /* lognormal coefficient of variation, 0 = const  */
+    double tau_c_cv;
+    double acut;              /* neighbour cutoff a_farcut                       */
+    double dt;                /* time step; <= 0 derives it (rf_derive)          */
+    uint64_t strength_seed;   /* seed of the per-bond strength draw              */
+    uint32_t quirks;          /* RF_QUIRK_* bitmask                              */
+    uint32_t pad_;
+} rf_material;
+
+/* Space box, hidden point and cell grid (sample_data / cell_data,
+ * md_data_types.h:24-124).  Particles leaving center +- half are moved to the
+ * hidden point (md_parallelepipedrestriction.cu:73-160).  The cell grid has
+ * its origin at 0, edge `cell` and dims[0..2] cells; every particle must lie in
+ * cells 1..dims-2 on each axis. */
+typedef struct rf_box {
+    double center[3];
+    double half[3];
+    double hidden[3];
+    double cell;
+    int32_t dims[3];
+    int32_t pad_;
+} rf_box;
+
+/* A sample: N spheres, SoA coordinates and radii (reduced units). */
+typedef struct rf_sample {
+    int64_t n;
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* r;
+} rf_sample;
+
+typedef struct rf_options {
+    int32_t slots;        /* interaction slots per particle (IonP), default 32  */
+    int32_t cand_cap;     /* neighbour-candidate capacity per particle (GPU)   */
+    double skin;          /* candidate-list skin width (GPU); never changes results */
+    int32_t device;       /* HIP device ordinal                                */
+    int32_t block;        /* threads per block of the per-particle kernels     */
+} rf_options;
+
+/* Derived parameters (potential_data::CalculateParameters, md_data_types.h:155-187). */
+typedef struct rf_derived {
+    double G_bond, G_contact, hm_E, hm_G, p_A;
+    double T_rayleigh, T_bond, dt;
+    double r_min, r_max;
+} rf_derived;
+
+typedef enum rf_test_kind {
+    RF_TEST_NONE = 0,      /* no platens: free DEM steps                    */
+    RF_TEST_UNIAXIAL = 1,  /* z platens, CompressU_S3 controller            */
+    RF_TEST_BRAZILIAN = 2  /* y platens, CompressB_S3 controller            */
+} rf_test_kind;
+
+/* Platen test configuration (compression_data::CalculateUParameters /
+ * CalculateBParameters, md_data_types.h:212-242, 380-409; driver values at
+ * mp_calculateonesample.cu:120-124). */
+typedef struct rf_platen_cfg {
+    int32_t kind;              /* rf_test_kind                                     */
+    int32_t dnsave;            /* curve/bond-count sampling interval (dNsave)      */
+    double v_load;             /* VLoad, negative compresses                       */
+    double min_stress;         /* MinStress                                        */
+    double max_strain;         /* MaxStrain                                        */
+    double cond_fracture;      /* ConditionFracture                                */
+    double size[3];            /* specimen size_real                               */
+    double center[3];          /* specimen center_real                             */
+    double r_ref;              /* radius used in the area formula (p_r)           */
+    double wall_modulus_factor;/* 2 (uniaxial) or 1 (Brazilian) times hm_E, hm_G   */
+    int64_t max_steps;         /* hard cap on loading steps (<=0: none)            */
+    int64_t curve_cap;         /* capacity of the curve buffer (<=0: derived)      */
+} rf_platen_cfg;
+
+/* Controller and macro state (compression_data + result_data fields). */
+typedef struct rf_test_state {
+    double top, bottom, v, size_axis, size0, strain, stress;
+    double peak_stress, peak_strain, sfl0, sfl1;
+    double linearity_deviation, destruction;
+    int64_t step;              /* loading steps executed                          */
+    int64_t steps_stop;        /* StepsStop (paused-platen counter)                */
+    int64_t n_curve;           /* stepsave                                         */
+    int32_t loading, fracture, done, kind;
+} rf_test_state;
+
+/* Per-step counters: the sizes the algorithmic-bytes model is built from. */
+typedef struct rf_counters {
+    int64_t n;            /* particles                                            */
+    int64_t occupied;     /* P: occupied directed slots                           */
+    int64_t bonds;        /* Bd: directed bond slots                              */
+    int64_t contacts;     /* Co: directed unbonded slots in overlap               */
+    int64_t steps;        /* loading steps executed so far                        */
+    int64_t rebuilds;     /* candidate-list rebuilds so far                       */
+    int64_t candidates;   /* stored neighbour candidates (sum over particles)     */
+} rf_counters;
+
+/* Full per-particle and per-slot state, caller-owned buffers (any may be NULL
+ * in rf_get_state to skip).  Particle arrays hold 3*n doubles (x block,
+ * y block, z block).  Slot arrays are row-major [i*slots + k]; vectors per
+ * slot are 3 consecutive doubles [(i*slots + k)*3 + c]. */
+typedef struct rf_state_view {
+    double* R; double* V; double* W; double* F; double* M;
+    int32_t* nbr;         /* partner id, -1 = empty                                */
+    uint8_t* type;        /* 1 = bond, 0 = contact slot                            */
+    double* inv_L0;       /* 1/L0 per slot                                         */
+    double* prev_n;       /* bond previous normal                                  */
+    double* o_t;          /* tangential overlap / shear displacement (bond and contact) */
+    double* m_n;          /* bond torsion history                                  */
+    double* m_t;          /* bond bending history                                  */
+    double* sigma_c;      /* per-slot bond strengths                               */
+    double* tau_c;
+    int8_t* bidx;         /* bond's position in the particle's force sum (its number
+                             in ascending slot order at creation), -1 otherwise     */
+} rf_state_view;
+
+/* Opaque context: one sample, all device state. */
+typedef struct rf_ctx rf_ctx;
+
+const char* rf_last_error(void);
+int32_t rf_version(void);
+
+/* Derived parameters for a sample with radii in [r_min, r_max]. */
+rf_status rf_derive(const rf_material* mat, double r_min, double r_max, rf_derived* out);
+
+/* Reference grid set-up: space box of size `space` (sample_data::SetSpaceSizeSi,
+ * CalculateParameters, md_data_types.h:87-124; cell_data::CalculateParameters
+ * :42-55) with cell edge `cell` (= 2*acut).  Fills center, half, hidden, dims. */
+rf_status rf_box_reference(const double space[3], double cell, rf_box* out);
+
+rf_status rf_create(const rf_sample* sample, const rf_material* mat, const rf_box* box,
+                    const rf_options* opt, rf_ctx** out);
+rf_status rf_destroy(rf_ctx* ctx);
+
+/* t=0 set-up of a loading run: cell sort, interaction rows
+ * (d_ConstructInteractionList), bonds (d_ConstructBoundInteractions), per-bond
+ * strengths and the first prune (d_DeleteFarLinks);
+ * md_calculationuniaxialcompression.cu:155-158, 212. */
+rf_status rf_build_bonds(rf_ctx* ctx);
+
+/* Select the platen test (or RF_TEST_NONE) and reset the controller. */
+rf_status rf_set_test(rf_ctx* ctx, const rf_platen_cfg* cfg);
+
+/* Advance up to nsteps loading steps (stops early when the test ends).
+ * *done_steps receives the number of steps executed. */
+rf_status rf_step(rf_ctx* ctx, int64_t nsteps, int64_t* done_steps);
+
+/* Run the selected test to its end (strain limit, fracture or max_steps) and
+ * post-process (CalculateLinearityDeviation, CalculateDestruction). */
+rf_status rf_run(rf_ctx* ctx, rf_test_state* out);
+
+rf_status rf_get_test_state(rf_ctx* ctx, rf_test_state* out);
+/* Curve samples: 3 doubles each (strain, bottom stress, top stress), as
+ * SaveSumForcesLoading (md_results.cu:168-180) in reduced units. */
+rf_status rf_get_curve(rf_ctx* ctx, double* out, int64_t cap, int64_t* n);
+/* Bond-count samples (d_SumUpInteractions, md_results.cu:189-272). */
+rf_status rf_get_bond_counts(rf_ctx* ctx, int64_t* out, int64_t cap, int64_t* n);
+
+rf_status rf_get_counters(rf_ctx* ctx, rf_counters* out);
+rf_status rf_get_state(rf_ctx* ctx, rf_state_view* out);
+rf_status rf_set_state(rf_ctx* ctx, const rf_state_view* in);
+int64_t rf_num_particles(const rf_ctx* ctx);
+int32_t rf_num_slots(const rf_ctx* ctx);
+
+/* Timing: with timing enabled, rf_step brackets every launch of the fused
+ * step kernel with HIP events on the context's stream.  rf_last_timing
+ * returns the summed kernel time, the wall time of the last rf_step call
+ * (both ms) and the number of steps it executed. */
+rf_status rf_set_timing(rf_ctx* ctx, int32_t enable);
+rf_status rf_last_timing(rf_ctx* ctx, double* step_kernel_ms, double* total_ms, int64_t* steps);
+
+/* Steps per host synchronisation inside rf_step (default 256). */
+rf_status rf_set_chunk(rf_ctx* ctx, int32_t steps_per_sync);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ROCKFIT_DEM_H */

@@ -1,0 +1,113 @@
+"""GPU (HIP) path against the CPU FP64 oracle on identical seeded samples.
+
+Bar: interaction lists, bond numbering and the bond-break sequence identical;
+particle state, forces, moments and histories bitwise identical (the two
+sides follow one floating-point operation order); controller state and
+curves identical.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from helpers import compare_states, compare_test_states, randomize_motion
+from rockfit import abi, config
+from rockfit.core import Context
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(w, slots=32, skin=-1.0):
+    g = Context(w["xyz"], w["r"], w["mat"], w["box"], slots=slots, skin=skin)
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], slots=slots, threads=4)
+    return g, o
+
+
+@pytest.fixture(scope="module")
+def small_uniaxial():
+    return config.make_workload(n=2000, r_min=0.2, r_max=0.3, test="uniaxial",
+                                test_kwargs=dict(v_load_mps=-2.0, dnsave=5))
+
+
+def test_build_parity(small_uniaxial):
+    w = small_uniaxial
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    gs, os_ = g.get_state(), o.get_state()
+    compare_states(gs, os_, g.n, g.slots, what="after build")
+    assert np.count_nonzero(os_["type"]) > 3 * g.n  # a bonded sample
+
+
+def test_uniaxial_steps_parity(small_uniaxial):
+    w = small_uniaxial
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    done = 0
+    for chunk in (1, 9, 90, 200, 300):
+        kg = g.step(chunk)
+        ko = o.step(chunk)
+        assert kg == ko
+        done += kg
+        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"step {done}")
+        compare_test_states(g.test_state(), o.test_state(), what=f"step {done}")
+    ts = g.test_state()
+    assert ts.stress > 0.0, "platens never reached the specimen"
+    np.testing.assert_array_equal(g.curve(), o.curve())
+    np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
+
+
+def test_free_motion_append_parity():
+    """Random velocities, no platens: contacts open and close, rows are
+    appended and pruned every step; small skin forces frequent rebuilds."""
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="none",
+                             mat_kwargs=dict(sigma_mpa=5.0, tau_mpa=8.0))
+    g, o = _pair(w, skin=0.02)
+    g.build_bonds()
+    o.build_bonds()
+    st = randomize_motion(o.get_state(), g.n, 7, 2e-3, 2e-2)
+    g.set_state(st)
+    o.set_state(st)
+    cfg = abi.PlatenCfg()
+    cfg.kind = abi.RF_TEST_NONE
+    g.set_test(cfg)
+    o.set_test(cfg)
+    for chunk in (1, 20, 80):
+        assert g.step(chunk) == o.step(chunk) == chunk
+        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"free motion +{chunk}")
+    c = g.counters()
+    assert c.rebuilds > 2
+
+
+def test_breakage_parity():
+    """Weak bonds under fast loading: the break sequence must match."""
+    w = config.make_workload(n=2000, r_min=0.2, r_max=0.3, test="uniaxial",
+                             mat_kwargs=dict(sigma_mpa=4.0, tau_mpa=6.0),
+                             test_kwargs=dict(v_load_mps=-4.0, dnsave=1, max_steps=500))
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    assert g.step(500) == o.step(500)
+    bg, bo = g.bond_counts(), o.bond_counts()
+    np.testing.assert_array_equal(bg, bo)
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="breakage")
+    compare_test_states(g.test_state(), o.test_state(), what="breakage")
+
+
+def test_brazilian_parity():
+    w = config.make_workload(n=2000, r_min=0.2, r_max=0.3, shape="disc", aspect=0.5,
+                             test="brazilian", test_kwargs=dict(v_load_mps=-2.0, dnsave=5,
+                                                                max_steps=400))
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    assert g.step(400) == o.step(400)
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="brazilian")
+    compare_test_states(g.test_state(), o.test_state(), what="brazilian")
+    np.testing.assert_array_equal(g.curve(), o.curve())
