@@ -61,7 +61,7 @@ def _container_push(p, r, R, H):
 
 
 def generate(n, r_min, r_max, shape="cylinder", aspect=2.0, packing=0.60, seed=20261203,
-             tol=1e-2, max_iter=4000, jitter=0.15, step=0.5, verbose=False):
+             tol=0.02, max_iter=300, jitter=0.05, step=0.5, verbose=False):
     """Return (xyz[n,3], r[n]) in mm, centred at the origin.
 
     shape "cylinder": height = aspect * diameter (axis z).

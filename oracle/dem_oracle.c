@@ -83,6 +83,22 @@ static void set_err(ora_ctx* c, const char* m) {
 
 const char* ora_error(const ora_ctx* c) { return c ? c->err : "null context"; }
 
+/* sizeof of the public structs as the C compiler lays them out (ABI check). */
+int64_t ora_sizeof(int32_t which) {
+    switch (which) {
+        case 0: return (int64_t)sizeof(rf_material);
+        case 1: return (int64_t)sizeof(rf_box);
+        case 2: return (int64_t)sizeof(rf_sample);
+        case 3: return (int64_t)sizeof(rf_options);
+        case 4: return (int64_t)sizeof(rf_derived);
+        case 5: return (int64_t)sizeof(rf_platen_cfg);
+        case 6: return (int64_t)sizeof(rf_test_state);
+        case 7: return (int64_t)sizeof(rf_counters);
+        case 8: return (int64_t)sizeof(rf_state_view);
+        default: return -1;
+    }
+}
+
 void ora_set_threads(ora_ctx* c, int32_t nthreads) {
     c->nthreads = nthreads > 0 ? nthreads : 1;
 }

@@ -27,6 +27,7 @@ ora_ctx* ora_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
                     int32_t slots, int32_t nthreads);
 void ora_destroy(ora_ctx* c);
 const char* ora_error(const ora_ctx* c);
+int64_t ora_sizeof(int32_t which);
 void ora_set_threads(ora_ctx* c, int32_t nthreads);
 
 /* Cell keys, stable (cell, id) order, cell start/count (CellDistribution). */

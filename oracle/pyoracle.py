@@ -74,6 +74,8 @@ def lib():
     L.ora_bond_strength.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_double,
                                     C.c_double]
     L.ora_bond_strength.restype = C.c_double
+    L.ora_sizeof.argtypes = [C.c_int32]
+    L.ora_sizeof.restype = C.c_int64
     _lib = L
     return L
 

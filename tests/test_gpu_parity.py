@@ -25,7 +25,7 @@ def _pair(w, slots=32, skin=-1.0):
 @pytest.fixture(scope="module")
 def small_uniaxial():
     return config.make_workload(n=2000, r_min=0.2, r_max=0.3, test="uniaxial",
-                                test_kwargs=dict(v_load_mps=-2.0, dnsave=5))
+                                test_kwargs=dict(v_load_mps=-5.0, dnsave=5))
 
 
 def test_build_parity(small_uniaxial):
@@ -53,9 +53,9 @@ def test_uniaxial_steps_parity(small_uniaxial):
         done += kg
         compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"step {done}")
         compare_test_states(g.test_state(), o.test_state(), what=f"step {done}")
-    ts = g.test_state()
-    assert ts.stress > 0.0, "platens never reached the specimen"
-    np.testing.assert_array_equal(g.curve(), o.curve())
+    curve = g.curve()
+    assert np.any(curve[:, 2] != 0.0), "platens never reached the specimen"
+    np.testing.assert_array_equal(curve, o.curve())
     np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
 
 
