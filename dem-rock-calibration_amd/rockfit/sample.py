@@ -124,11 +124,16 @@ def generate(n, r_min, r_max, shape="cylinder", aspect=2.0, packing=0.60, seed=2
 
 
 def cylinder_geometry(xyz, r):
-    """Specimen size and centre (size_real / center_real as h_LoadCylinderSample,
-    md_createcylindersample.cu:448-465, with r_max in place of p_r)."""
-    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
-    rmax = float(r.max())
-    size = hi - lo + 2.0 * rmax
+    """Specimen size and centre: the box around all sphere surfaces.
+
+    h_LoadCylinderSample (md_createcylindersample.cu:448-465) takes the
+    extent of the centres plus 2 p_r; with one radius that is exactly the
+    envelope min(x - r) .. max(x + r) used here, which keeps the platens
+    touching the outermost spheres for any radius distribution."""
+    rr = np.asarray(r, dtype=np.float64)[:, None]
+    lo = (xyz - rr).min(axis=0)
+    hi = (xyz + rr).max(axis=0)
+    size = hi - lo
     center = 0.5 * (hi + lo)
     return size, center
 

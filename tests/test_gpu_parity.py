@@ -25,7 +25,7 @@ def _pair(w, slots=32, skin=-1.0):
 @pytest.fixture(scope="module")
 def small_uniaxial():
     return config.make_workload(n=2000, r_min=0.2, r_max=0.3, test="uniaxial",
-                                test_kwargs=dict(v_load_mps=-5.0, dnsave=5))
+                                test_kwargs=dict(v_load_mps=-2.0, dnsave=5, max_strain=0.05))
 
 
 def test_build_parity(small_uniaxial):
@@ -84,8 +84,9 @@ def test_free_motion_append_parity():
 def test_breakage_parity():
     """Weak bonds under fast loading: the break sequence must match."""
     w = config.make_workload(n=2000, r_min=0.2, r_max=0.3, test="uniaxial",
-                             mat_kwargs=dict(sigma_mpa=4.0, tau_mpa=6.0),
-                             test_kwargs=dict(v_load_mps=-4.0, dnsave=1, max_steps=500))
+                             mat_kwargs=dict(sigma_mpa=8.0, tau_mpa=12.0),
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=1, max_steps=500,
+                                              max_strain=0.05))
     g, o = _pair(w)
     g.build_bonds()
     o.build_bonds()
@@ -94,6 +95,8 @@ def test_breakage_parity():
     assert g.step(500) == o.step(500)
     bg, bo = g.bond_counts(), o.bond_counts()
     np.testing.assert_array_equal(bg, bo)
+    assert bg[-1] < bg[0] // 2, "expected the specimen to fail"
+    assert g.test_state().fracture == 1
     compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="breakage")
     compare_test_states(g.test_state(), o.test_state(), what="breakage")
 
