@@ -894,14 +894,20 @@ __global__ void k_counters(DevState s, DevParams p, Ctrl* ctrl, unsigned long lo
     if (i >= p.n) return;
     const double4* pos = s.pos[ctrl->cur];
     const double4 pi = pos[i];
-    const uint32_t occ = s.occ[i], bnd = s.bnd[i];
+    uint32_t occ = s.occ[i], bnd = s.bnd[i];
     unsigned long long co = 0;
     for (int k = 0; k < p.S; ++k) {
-        if (!((occ >> k) & 1u) || ((bnd >> k) & 1u)) continue;
+        if (!((occ >> k) & 1u)) continue;
         const double4 pj = pos[s.nbr[i * p.S + k]];
         const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double rr = dx * dx + dy * dy + dz * dz;
+        if (rr > p.acut2 || rr < 1e-10) {  // pruned at the start of the next step
+            occ &= ~(1u << k);
+            bnd &= ~(1u << k);
+            continue;
+        }
         const double rs = pi.w + pj.w;
-        if (dx * dx + dy * dy + dz * dz < rs * rs) ++co;
+        if (!((bnd >> k) & 1u) && rr < rs * rs) ++co;
     }
     atomicAdd(&out[0], (unsigned long long)__popc(occ));
     atomicAdd(&out[1], (unsigned long long)__popc(bnd));

@@ -769,6 +769,22 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
         bf[f].resize(n * B);
         HIP_TRY(hipMemcpy(bf[f].data(), bsrc[f], n * B * 8, hipMemcpyDeviceToHost));
     }
+    // The device prunes a slot at the start of the next step; the reference
+    // prunes it at the end of this one (d_DeleteFarLinks).  Report the state
+    // after that end-of-step prune, with the kernel's predicate.
+    const double acut2 = c->p.acut2;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t k = 0; k < S; ++k) {
+            if (!((occ[i] >> k) & 1u)) continue;
+            const int64_t j = nbr[i * S + k];
+            const double dx = hp[j].x - hp[i].x, dy = hp[j].y - hp[i].y, dz = hp[j].z - hp[i].z;
+            const double rr = dx * dx + dy * dy + dz * dz;
+            if (rr > acut2 || rr < 1e-10) {
+                occ[i] &= ~(1u << k);
+                bnd[i] &= ~(1u << k);
+            }
+        }
+    }
     for (int64_t i = 0; i < n; ++i) {
         for (int64_t k = 0; k < S; ++k) {
             const int64_t q = i * S + k;

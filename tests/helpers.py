@@ -28,8 +28,10 @@ def compare_states(g, o, n, S, float_tol=0.0, what=""):
             errs.append(f"{f}: max rel diff {r:.3e}")
     if not np.array_equal(g["nbr"], o["nbr"]):
         bad = np.nonzero(g["nbr"] != o["nbr"])[0]
-        errs.append(f"nbr differs at {len(bad)} slots, first slot {bad[0]} "
-                    f"(particle {bad[0] // S}): gpu {g['nbr'][bad[0]]} oracle {o['nbr'][bad[0]]}")
+        rows = sorted({int(q) // S for q in bad})[:4]
+        detail = "; ".join(f"particle {p}: gpu {g['nbr'][p * S:(p + 1) * S].tolist()} "
+                           f"oracle {o['nbr'][p * S:(p + 1) * S].tolist()}" for p in rows)
+        errs.append(f"nbr differs at {len(bad)} slots: {detail}")
     if not np.array_equal(g["type"], o["type"]):
         bad = np.nonzero(g["type"] != o["type"])[0]
         errs.append(f"type differs at {len(bad)} slots")

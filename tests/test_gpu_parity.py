@@ -74,9 +74,22 @@ def test_free_motion_append_parity():
     cfg.kind = abi.RF_TEST_NONE
     g.set_test(cfg)
     o.set_test(cfg)
+    prev = o.get_state()
     for chunk in (1, 20, 80):
         assert g.step(chunk) == o.step(chunk) == chunk
-        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"free motion +{chunk}")
+        gs, os_ = g.get_state(), o.get_state()
+        try:
+            compare_states(gs, os_, g.n, g.slots, what=f"free motion +{chunk}")
+        except AssertionError:
+            import os
+            out = os.path.join(os.environ.get("GRAFT_REPO_ROOT", "."), "gpurun_out")
+            if os.path.isdir(out):
+                np.savez_compressed(os.path.join(out, "free_motion_diverge.npz"),
+                                    g_nbr=gs["nbr"], o_nbr=os_["nbr"], R=os_["R"], V=os_["V"],
+                                    W=os_["W"], prev_R=prev["R"], prev_nbr=prev["nbr"],
+                                    xyz=w["xyz"], r=w["r"])
+            raise
+        prev = os_
     c = g.counters()
     assert c.rebuilds > 2
 
