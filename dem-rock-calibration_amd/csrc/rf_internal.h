@@ -15,13 +15,25 @@ constexpr int kBlock = 256;
 constexpr int kPerBlock = kBlock / kGroup;  // 8 particles per block
 constexpr int kMaxSlots = 32;               // slots per particle (one lane each)
 constexpr int kMaxCand = 64;                // candidates per particle (two per lane)
-constexpr int kCtrlThreads = 256;           // threads of the controller block
+constexpr int kCtrlThreads = 1024;          // threads of the controller block
 
 // Error bits raised by device code (Ctrl::err).
 constexpr int kErrRowOverflow = 1;   // no free slot for a new contact
 constexpr int kErrCandOverflow = 2;  // candidate list longer than cand_cap
 constexpr int kErrGrid = 4;          // particle outside the padded cell grid
 constexpr int kErrCurve = 8;         // curve buffer full
+
+// One bond of a particle's bond table: 128 bytes, the 12 history values
+// first (written back each step), then the constants.
+struct BondRec {
+    double pnx, pny, pnz;   // previous bond normal
+    double otx, oty, otz;   // shear displacement
+    double mnx, mny, mnz;   // torsion history
+    double mtx, mty, mtz;   // bending history
+    double invL0, sc, tc;   // 1/L0, normal and shear strength
+    double pad;
+};
+static_assert(sizeof(BondRec) == 128, "bond record must be one 128-byte line");
 
 // Device-resident arrays.  Particles are array-of-structs double4 so a
 // neighbour gather is one 32-byte load:
@@ -39,18 +51,15 @@ struct DevState {
     double4* rbuild;   // positions at the last candidate rebuild
     // slot table
     int32_t* nbr;      // partner of slot k (valid where the occupancy bit is set)
-    double* otx; double* oty; double* otz;  // contact tangential overlap
+    double4* ot;       // contact tangential overlap (x, y, z, 0)
     uint32_t* occ;     // occupied slots
     uint32_t* bnd;     // bond slots
     uint32_t* otnz;    // contact slots whose o_t may be non-zero
     // bond table (bonds numbered in ascending slot order at creation)
     int32_t* nbond;
-    int8_t* bslot; int32_t* bpart;
-    double* binvL0; double* bsc; double* btc;
-    double* bpnx; double* bpny; double* bpnz;
-    double* botx; double* boty; double* botz;
-    double* bmnx; double* bmny; double* bmnz;
-    double* bmtx; double* bmty; double* bmtz;
+    int8_t* bslot;     // slot of bond b, -1 = none
+    int32_t* bpart;    // partner of bond b
+    BondRec* brec;
     // candidate lists
     int32_t* cand; int8_t* cslot; int32_t* ncand;
     // cell sort
@@ -101,6 +110,8 @@ hipError_t launch_construct_bonds(const DevState& s, const DevParams& p, Ctrl* c
 hipError_t launch_prune_initial(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_build_candidates(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_clear_flags(Ctrl* ctrl, hipStream_t st);
+hipError_t launch_set_strengths(BondRec* brec, const double* sc, const double* tc, int64_t n,
+                                hipStream_t st);
 hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_counters(const DevState& s, const DevParams& p, Ctrl* ctrl, int64_t* out3,

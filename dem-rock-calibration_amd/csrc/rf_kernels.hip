@@ -87,6 +87,8 @@ __device__ __forceinline__ int free_slot(uint32_t freem, int r, bool q0) {
     return r < __popc(freem) ? nth_bit(freem, r) : -1;
 }
 
+__device__ __forceinline__ double ri_of(const double4& p) { return p.w; }
+
 struct WallOut {
     double fnx, fny, fnz, ftx, fty, ftz, mtx, mty, mtz, max_, may, maz, ox, oy, oz;
 };
@@ -147,6 +149,9 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 }
 
 // ------------------------------------------------------------ step kernel --
+// Memory round trips per group: (1) everything indexed by the particle,
+// (2) one batch of gathers with clamped indices (slot partners, candidates,
+// bond partners and bond records), (3) velocities of overlapping contacts.
 __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* __restrict__ ctrl) {
     __shared__ double sh_b[kPerBlock];
     __shared__ double sh_t[kPerBlock];
@@ -166,23 +171,46 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
 
     if (valid) {
         const int S = p.S, C = p.C, B = p.B;
+        // ---- round trip 1: rows of particle i
         const double4 pi = pos[i];
         const double4 vi = vel[i];
         const double4 wi = angv[i];
-        const double ri = pi.w, mi = vi.w;
-        const uint32_t slotmask = S >= 32 ? 0xffffffffu : ((1u << S) - 1u);
         uint32_t occ = s.occ[i], bnd = s.bnd[i], otnz = s.otnz[i];
-
-        // ---- 1a. prune: slot lanes re-check their partner (d_DeleteFarLinks)
+        const int nc = s.ncand[i];
+        const int nbd = s.nbond[i];
         const int k = lane;
         const int64_t qs = i * S + k;
+        const int32_t jk_raw = k < S ? s.nbr[qs] : 0;
+        const int64_t qc0 = i * C + lane, qc1 = qc0 + 32;
+        const bool in0 = lane < nc, in1 = lane + 32 < nc;
+        const int32_t cj0 = in0 ? s.cand[qc0] : 0;
+        const int32_t cj1 = in1 ? s.cand[qc1] : 0;
+        const int cs0 = in0 ? (int)s.cslot[qc0] : -1;
+        const int cs1 = in1 ? (int)s.cslot[qc1] : -1;
+        const int64_t qb = i * B + lane;
+        const bool inb = lane < nbd;
+        const int32_t bj_raw = inb ? s.bpart[qb] : 0;
+        const int bks = inb ? (int)s.bslot[qb] : -1;
+
+        // ---- round trip 2: gathers (clamped to particle i where unused)
         const bool had = k < S && ((occ >> k) & 1u);
-        int32_t jk = 0;
-        double4 pjk = make_double4(0.0, 0.0, 0.0, 0.0);
+        const int32_t jk = had ? jk_raw : (int32_t)i;
+        const bool ab0 = in0 && cs0 < 0, ab1 = in1 && cs1 < 0;
+        const bool bl0 = inb && bks >= 0 && ((bnd >> bks) & 1u);
+        const int32_t bj = bl0 ? bj_raw : (int32_t)i;
+        const double4 pjk = pos[jk];
+        const double4 pc0 = pos[ab0 ? cj0 : (int32_t)i];
+        const double4 pc1 = pos[ab1 ? cj1 : (int32_t)i];
+        const double4 pbj = pos[bj];
+        const double4 vbj = vel[bj];
+        const double4 wbj = angv[bj];
+        const double2* rp = reinterpret_cast<const double2*>(s.brec + (bl0 ? qb : i * B));
+        const double2 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3], r4 = rp[4], r5 = rp[5],
+                      r6 = rp[6], r7 = rp[7];
+
+        // ---- 1a. prune: slot lanes re-check their partner (d_DeleteFarLinks)
         bool prn = false;
         if (had) {
-            jk = s.nbr[qs];
-            pjk = pos[jk];
             const double dx = pjk.x - pi.x, dy = pjk.y - pi.y, dz = pjk.z - pi.z;
             const double rr = dx * dx + dy * dy + dz * dz;
             prn = rr > p.acut2 || rr < 1e-10;
@@ -193,7 +221,6 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         otnz &= ~prune;
 
         // ---- 1b. append: candidate lanes (d_ReConstructInteractionListbpm)
-        const int nc = s.ncand[i];
         const double fx = floor(pi.x * p.inv_cell), fy = floor(pi.y * p.inv_cell),
                      fz = floor(pi.z * p.inv_cell);
         const double sx = copysign(1.0, pi.x - (fx + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
@@ -201,38 +228,34 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         const double sz = copysign(1.0, pi.z - (fz + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
         bool nw0 = false, nw1 = false;
         uint64_t key0 = 0, key1 = 0;
-        int32_t jn0 = 0, jn1 = 0;
+        if (cs0 >= 0 && ((prune >> cs0) & 1u)) s.cslot[qc0] = -1;
+        if (cs1 >= 0 && ((prune >> cs1) & 1u)) s.cslot[qc1] = -1;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int c = lane + 32 * h;
-            if (c < nc) {
-                const int64_t qc = i * C + c;
-                const int cs = s.cslot[qc];
-                if (cs >= 0) {
-                    if ((prune >> cs) & 1u) s.cslot[qc] = -1;
-                } else {
-                    const int32_t j = s.cand[qc];
-                    const double4 pj = pos[j];
-                    const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                    const double rr = dx * dx + dy * dy + dz * dz;
-                    if (rr < p.acut2 && rr > 1e-10) {
-                        const double gx = floor(pj.x * p.inv_cell), gy = floor(pj.y * p.inv_cell),
-                                     gz = floor(pj.z * p.inv_cell);
-                        const int bx = gx == fx ? 0 : (gx == fx + sx ? 1 : -1);
-                        const int by = gy == fy ? 0 : (gy == fy + sy ? 1 : -1);
-                        const int bz = gz == fz ? 0 : (gz == fz + sz ? 1 : -1);
-                        if (bx >= 0 && by >= 0 && bz >= 0) {
-                            const uint64_t key =
-                                ((uint64_t)kStencilPos[bx | (by << 1) | (bz << 2)] << 32) |
-                                (uint32_t)j;
-                            if (h == 0) { nw0 = true; key0 = key; jn0 = j; }
-                            else { nw1 = true; key1 = key; jn1 = j; }
-                        }
+            const bool ab = h == 0 ? ab0 : ab1;
+            if (ab) {
+                const double4 pj = h == 0 ? pc0 : pc1;
+                const int32_t j = h == 0 ? cj0 : cj1;
+                const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const double rr = dx * dx + dy * dy + dz * dz;
+                if (rr < p.acut2 && rr > 1e-10) {
+                    const double gx = floor(pj.x * p.inv_cell), gy = floor(pj.y * p.inv_cell),
+                                 gz = floor(pj.z * p.inv_cell);
+                    const int bx = gx == fx ? 0 : (gx == fx + sx ? 1 : -1);
+                    const int by = gy == fy ? 0 : (gy == fy + sy ? 1 : -1);
+                    const int bz = gz == fz ? 0 : (gz == fz + sz ? 1 : -1);
+                    if (bx >= 0 && by >= 0 && bz >= 0) {
+                        const uint64_t key =
+                            ((uint64_t)kStencilPos[bx | (by << 1) | (bz << 2)] << 32) | (uint32_t)j;
+                        if (h == 0) { nw0 = true; key0 = key; }
+                        else { nw1 = true; key1 = key; }
                     }
                 }
             }
         }
         const uint32_t nb0 = group_ballot(nw0), nb1 = group_ballot(nw1);
+        uint32_t added = 0u;   // slots filled this step
+        int32_t jnew = 0;      // partner of this lane's slot if it was filled this step
         if ((nb0 | nb1) != 0u) {
             // Rank the new pairs by visiting order (stencil cell, then id) and
             // give them the free slots in fill order.
@@ -254,114 +277,122 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
                 if (nw1 && kb < key1) ++rank1;
             }
             const int total = __popc(nb0) + __popc(nb1);
-            const uint32_t freem = ~occ & slotmask;
+            const uint32_t freem = ~occ & (S >= 32 ? 0xffffffffu : ((1u << S) - 1u));
             const bool q0 = (p.quirks & RF_QUIRK_SLOT0_SENTINEL) && i == 0;
             if (total > __popc(freem) && lane == 0) atomicOr(&ctrl->err, kErrRowOverflow);
-            if (nw0) {
-                const int sl = free_slot(freem, rank0, q0);
+            const int sl0 = nw0 ? free_slot(freem, rank0, q0) : -1;
+            const int sl1 = nw1 ? free_slot(freem, rank1, q0) : -1;
+            if (sl0 >= 0) {
+                s.nbr[i * S + sl0] = cj0;
+                s.cslot[qc0] = (int8_t)sl0;
+            }
+            if (sl1 >= 0) {
+                s.nbr[i * S + sl1] = cj1;
+                s.cslot[qc1] = (int8_t)sl1;
+            }
+            // hand each filled slot's partner to the slot's lane
+            m = nb0;
+            while (m) {
+                const int L = __builtin_ctz(m);
+                m &= m - 1u;
+                const int sl = __shfl(sl0, L, 32);
+                const int32_t j = __shfl(cj0, L, 32);
                 if (sl >= 0) {
-                    s.nbr[i * S + sl] = jn0;
-                    s.cslot[i * C + lane] = (int8_t)sl;
+                    added |= 1u << sl;
+                    if (sl == k) jnew = j;
                 }
             }
-            if (nw1) {
-                const int sl = free_slot(freem, rank1, q0);
+            m = nb1;
+            while (m) {
+                const int L = __builtin_ctz(m);
+                m &= m - 1u;
+                const int sl = __shfl(sl1, L, 32);
+                const int32_t j = __shfl(cj1, L, 32);
                 if (sl >= 0) {
-                    s.nbr[i * S + sl] = jn1;
-                    s.cslot[i * C + lane + 32] = (int8_t)sl;
+                    added |= 1u << sl;
+                    if (sl == k) jnew = j;
                 }
             }
-            uint32_t add = 0u;
-            for (int r = 0; r < total; ++r) {
-                const int sl = free_slot(freem, r, q0);
-                if (sl >= 0) add |= 1u << sl;
-            }
-            occ |= add;
+            occ |= added;
         }
 
         // ---- 2a. bonds: one lane per bond-table entry (d_CalculateForcesDEM_22)
         double cfx = 0.0, cfy = 0.0, cfz = 0.0, cmx = 0.0, cmy = 0.0, cmz = 0.0;
         uint32_t brk = 0u;
-        const int nbd = s.nbond[i];
-        if (lane < nbd) {
-            const int64_t qb = i * B + lane;
-            const int ks = s.bslot[qb];
-            if (ks >= 0 && ((bnd >> ks) & 1u)) {
-                const int32_t j = s.bpart[qb];
-                const double4 pj = pos[j];
-                const double4 vj = vel[j];
-                const double4 wj = angv[j];
-                const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-                const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-                const double inv = 1.0 / rijm;
-                const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
-                const double lam = s.binvL0[qb];
-                const double rj = pj.w;
-                const double b = p.b_ratio > 0.0 ? p.b_ratio * (ri < rj ? ri : rj) : p.b_abs;
-                double cefn = p.E * (rijm - 1.0 / lam) * lam;
-                double Sc = cefn;
-                cefn *= b * b * RF_PI;
-                const double pnx = s.bpnx[qb], pny = s.bpny[qb], pnz = s.bpnz[qb];
-                const double tx = pny * nz - pnz * ny;
-                const double ty = pnz * nx - pnx * nz;
-                const double tz = pnx * ny - pny * nx;
-                const double cw =
-                    ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
-                const double px = nx * cw, py = ny * cw, pz = nz * cw;
-                s.bpnx[qb] = nx; s.bpny[qb] = ny; s.bpnz[qb] = nz;
-                const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px,
-                             M1z = -py - tz * px + ty;
-                const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px,
-                             M2z = -tz * py + px - tx;
-                const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px,
-                             M3z = ty * py + tx * px + 1.0;
-                // torsion
-                const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
-                double c1 = nx * dwx + ny * dwy + nz * dwz;
-                c1 *= p.dt * 0.5 * p.G * lam;
-                const double on0 = s.bmnx[qb], on1 = s.bmny[qb], on2 = s.bmnz[qb];
-                const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
-                const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
-                const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
-                const double J4 = RF_PI * b * b * b * b;
-                // bending
-                const double c2 = p.dt * 0.25 * p.E * lam;
-                const double c1b = nx * dwx + ny * dwy + nz * dwz;
-                const double ot0 = s.bmtx[qb], ot1 = s.bmty[qb], ot2 = s.bmtz[qb];
-                const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
-                const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
-                const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
-                const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
-                // shear force and asymmetric moment
-                const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y),
-                             hz = 0.5 * (wj.z + wi.z);
-                const double vx = vj.x - vi.x - hy * rz + hz * ry;
-                const double vy = vj.y - vi.y - hz * rx + hx * rz;
-                const double vz = vj.z - vi.z - hx * ry + hy * rx;
-                const double cn = nx * vx + ny * vy + nz * vz;
-                const double oo0 = s.botx[qb], oo1 = s.boty[qb], oo2 = s.botz[qb];
-                const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
-                const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
-                const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
-                double cef = p.G * lam;
-                double Mc = cef;
-                cef *= b * b * RF_PI;
-                cfx = nx * cefn + ox * cef;
-                cfy = ny * cefn + oy * cef;
-                cfz = nz * cefn + oz * cef;
-                cmx = (mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef;
-                cmy = (mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef;
-                cmz = (mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef;
-                Mc *= sqrt(ox * ox + oy * oy + oz * oz);
-                Sc += sqrt(mtx * mtx + mty * mty + mtz * mtz) * 4.0 * b;
-                Mc += sqrt(mnx * mnx + mny * mny + mnz * mnz) * 2.0 * b;
-                if (Sc > s.bsc[qb] || Mc > s.btc[qb]) {
-                    brk = 1u << ks;  // the entry is dead from now on
-                } else {
-                    s.bmnx[qb] = mnx; s.bmny[qb] = mny; s.bmnz[qb] = mnz;
-                    s.bmtx[qb] = mtx; s.bmty[qb] = mty; s.bmtz[qb] = mtz;
-                    s.botx[qb] = ox; s.boty[qb] = oy; s.botz[qb] = oz;
-                }
+        if (bl0 && !((prune >> bks) & 1u)) {
+            const double4 pj = pbj, vj = vbj, wj = wbj;
+            const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+            const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
+            const double inv = 1.0 / rijm;
+            const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+            const double pnx = r0.x, pny = r0.y, pnz = r1.x;
+            const double oo0 = r1.y, oo1 = r2.x, oo2 = r2.y;
+            const double on0 = r3.x, on1 = r3.y, on2 = r4.x;
+            const double ot0 = r4.y, ot1 = r5.x, ot2 = r5.y;
+            const double lam = r6.x, bsc = r6.y, btc = r7.x;
+            const double rj = pj.w;
+            const double b = p.b_ratio > 0.0 ? p.b_ratio * (ri_of(pi) < rj ? ri_of(pi) : rj) : p.b_abs;
+            double cefn = p.E * (rijm - 1.0 / lam) * lam;
+            double Sc = cefn;
+            cefn *= b * b * RF_PI;
+            const double tx = pny * nz - pnz * ny;
+            const double ty = pnz * nx - pnx * nz;
+            const double tz = pnx * ny - pny * nx;
+            const double cw =
+                ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
+            const double px = nx * cw, py = ny * cw, pz = nz * cw;
+            const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px,
+                         M1z = -py - tz * px + ty;
+            const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px,
+                         M2z = -tz * py + px - tx;
+            const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px,
+                         M3z = ty * py + tx * px + 1.0;
+            // torsion
+            const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
+            double c1 = nx * dwx + ny * dwy + nz * dwz;
+            c1 *= p.dt * 0.5 * p.G * lam;
+            const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
+            const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
+            const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
+            const double J4 = RF_PI * b * b * b * b;
+            // bending
+            const double c2 = p.dt * 0.25 * p.E * lam;
+            const double c1b = nx * dwx + ny * dwy + nz * dwz;
+            const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
+            const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
+            const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
+            const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
+            // shear force and asymmetric moment
+            const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y), hz = 0.5 * (wj.z + wi.z);
+            const double vx = vj.x - vi.x - hy * rz + hz * ry;
+            const double vy = vj.y - vi.y - hz * rx + hx * rz;
+            const double vz = vj.z - vi.z - hx * ry + hy * rx;
+            const double cn = nx * vx + ny * vy + nz * vz;
+            const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
+            const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
+            const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
+            double cef = p.G * lam;
+            double Mc = cef;
+            cef *= b * b * RF_PI;
+            cfx = nx * cefn + ox * cef;
+            cfy = ny * cefn + oy * cef;
+            cfz = nz * cefn + oz * cef;
+            cmx = (mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef;
+            cmy = (mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef;
+            cmz = (mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef;
+            Mc *= sqrt(ox * ox + oy * oy + oz * oz);
+            Sc += sqrt(mtx * mtx + mty * mty + mtz * mtz) * 4.0 * b;
+            Mc += sqrt(mnx * mnx + mny * mny + mnz * mnz) * 2.0 * b;
+            if (Sc > bsc || Mc > btc) {
+                brk = 1u << bks;  // the entry is dead from now on
+            } else {
+                double2* wp = reinterpret_cast<double2*>(s.brec + qb);
+                wp[0] = make_double2(nx, ny);
+                wp[1] = make_double2(nz, ox);
+                wp[2] = make_double2(oy, oz);
+                wp[3] = make_double2(mnx, mny);
+                wp[4] = make_double2(mnz, mtx);
+                wp[5] = make_double2(mty, mtz);
             }
         }
         const double Fbx = tree32(cfx), Fby = tree32(cfy), Fbz = tree32(cfz);
@@ -369,21 +400,16 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         bnd &= ~group_or(brk);
 
         // ---- 2b. contacts: one lane per slot (d_CalculateForcesDEM_21)
+        const double ri = pi.w, mi = vi.w;
         const double wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
         const bool roll = wim > 1e-18;
         cfx = 0.0; cfy = 0.0; cfz = 0.0; cmx = 0.0; cmy = 0.0; cmz = 0.0;
         const bool isc = k < S && ((occ >> k) & 1u) && !((bnd >> k) & 1u);
         bool ovl = false;
         if (isc) {
-            int32_t j;
-            double4 pj;
-            if (had && !prn) {
-                j = jk;
-                pj = pjk;
-            } else {
-                j = s.nbr[qs];
-                pj = pos[j];
-            }
+            const bool fresh = (added >> k) & 1u;
+            const int32_t j = fresh ? jnew : jk;
+            const double4 pj = fresh ? pos[j] : pjk;
             const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
             const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
             const double inv = 1.0 / rijm;
@@ -394,6 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
                 ovl = true;
                 const double4 vj = vel[j];
                 const double4 wj = angv[j];
+                const double4 oprev = ((otnz >> k) & 1u) ? s.ot[qs] : make_double4(0.0, 0.0, 0.0, 0.0);
                 const double mj = vj.w;
                 const double Rst = ri * rj / (ri + rj);
                 const double meff = mi * mj / (mi + mj);
@@ -410,10 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
                 const double fn = (rijm - rsum) * Kn * kTwoThirds;
                 const double fd = kDamp * p.pA * vn * sqrt(Kn * meff);
                 const double vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
-                double o0 = 0.0, o1 = 0.0, o2 = 0.0;
-                if ((otnz >> k) & 1u) {
-                    o0 = s.otx[qs]; o1 = s.oty[qs]; o2 = s.otz[qs];
-                }
+                const double o0 = oprev.x, o1 = oprev.y, o2 = oprev.z;
                 c1 = nx * o0 + ny * o1 + nz * o2;
                 double ox = o0 - nx * c1, oy = o1 - ny * c1, oz = o2 - nz * c1;
                 c1 = ox * ox + oy * oy + oz * oz;
@@ -435,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
                     c1 = kDamp * p.pA * sqrt(Kt * meff);
                     ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
                 }
-                s.otx[qs] = ox; s.oty[qs] = oy; s.otz[qs] = oz;
+                s.ot[qs] = make_double4(ox, oy, oz, 0.0);
                 cfx = nx * (fn + fd) + ftx;
                 cfy = ny * (fn + fd) + fty;
                 cfz = nz * (fn + fd) + ftz;
@@ -502,25 +526,26 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
             part_b = -s_b;
             part_t = -s_t;
         }
-        const double2 pk = s.pk[i];
-        const double nvx = vi.x + Fx * pk.x, nvy = vi.y + Fy * pk.x, nvz = vi.z + Fz * pk.x;
-        double nrx = pi.x + nvx * p.dt, nry = pi.y + nvy * p.dt, nrz = pi.z + nvz * p.dt;
-        const double nwx_ = wi.x + Mx * pk.y, nwy_ = wi.y + My * pk.y, nwz_ = wi.z + Mz * pk.y;
-        double ovx = nvx, ovy = nvy, ovz = nvz;
-        bool out = false;
-        {
-            const double dx = nrx - p.cen[0], dy = nry - p.cen[1], dz = nrz - p.cen[2];
-            if (dx - p.half[0] > 0.0 || dx + p.half[0] < 0.0) out = true;
-            if (dy - p.half[1] > 0.0 || dy + p.half[1] < 0.0) out = true;
-            if (dz - p.half[2] > 0.0 || dz + p.half[2] < 0.0) out = true;
-        }
-        if (out) {
-            nrx = p.hid[0]; nry = p.hid[1]; nrz = p.hid[2];
-            ovx = 0.0; ovy = 0.0; ovz = 0.0;
-            Fx = 0.0; Fy = 0.0; Fz = 0.0;
-        }
         bonds_after = __popc(bnd);
         if (lane == 0) {
+            const double2 pk = s.pk[i];
+            const double4 rb = s.rbuild[i];
+            const double nvx = vi.x + Fx * pk.x, nvy = vi.y + Fy * pk.x, nvz = vi.z + Fz * pk.x;
+            double nrx = pi.x + nvx * p.dt, nry = pi.y + nvy * p.dt, nrz = pi.z + nvz * p.dt;
+            const double nwx_ = wi.x + Mx * pk.y, nwy_ = wi.y + My * pk.y, nwz_ = wi.z + Mz * pk.y;
+            double ovx = nvx, ovy = nvy, ovz = nvz;
+            bool out = false;
+            {
+                const double dx = nrx - p.cen[0], dy = nry - p.cen[1], dz = nrz - p.cen[2];
+                if (dx - p.half[0] > 0.0 || dx + p.half[0] < 0.0) out = true;
+                if (dy - p.half[1] > 0.0 || dy + p.half[1] < 0.0) out = true;
+                if (dz - p.half[2] > 0.0 || dz + p.half[2] < 0.0) out = true;
+            }
+            if (out) {
+                nrx = p.hid[0]; nry = p.hid[1]; nrz = p.hid[2];
+                ovx = 0.0; ovy = 0.0; ovz = 0.0;
+                Fx = 0.0; Fy = 0.0; Fz = 0.0;
+            }
             const int nxt = cur ^ 1;
             s.pos[nxt][i] = make_double4(nrx, nry, nrz, ri);
             s.vel[nxt][i] = make_double4(ovx, ovy, ovz, mi);
@@ -532,7 +557,6 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
             s.occ[i] = occ;
             s.bnd[i] = bnd;
             s.otnz[i] = otnz;
-            const double4 rb = s.rbuild[i];
             const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
             if (out || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
         }
@@ -570,6 +594,8 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
 // -------------------------------------------------------- controller ------
 // SumForcesULoading / SumForcesBLoading (md_results.cu:56-145) and
 // CompressU_S3 / CompressB_S3 (md_data_types.h:347-379, 462-494), on device.
+// Platen-force sum: kCtrlThreads strided running sums of the block partials,
+// then a kCtrlThreads-wide tree (the CPU restatement uses the same order).
 __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevParams p,
                                                              Ctrl* __restrict__ ctrl) {
     __shared__ double sb[kCtrlThreads];
@@ -579,6 +605,7 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevPara
     const int t = threadIdx.x;
     double ab = 0.0, at = 0.0;
     long long nb = 0;
+#pragma unroll 4
     for (int b = t; b < p.nblk; b += kCtrlThreads) {
         ab = ab + s.part_b[b];
         at = at + s.part_t[b];
@@ -792,11 +819,14 @@ __global__ void k_construct_bonds(DevState s, DevParams p, Ctrl* ctrl) {
             const int64_t qb = i * p.B + nb;
             s.bslot[qb] = (int8_t)k;
             s.bpart[qb] = j;
-            s.binvL0[qb] = 1.0 / sqrt(rm2);
-            s.bpnx[qb] = 0.0; s.bpny[qb] = 0.0; s.bpnz[qb] = 0.0;
-            s.botx[qb] = 0.0; s.boty[qb] = 0.0; s.botz[qb] = 0.0;
-            s.bmnx[qb] = 0.0; s.bmny[qb] = 0.0; s.bmnz[qb] = 0.0;
-            s.bmtx[qb] = 0.0; s.bmty[qb] = 0.0; s.bmtz[qb] = 0.0;
+            BondRec rec;
+            rec.pnx = 0.0; rec.pny = 0.0; rec.pnz = 0.0;
+            rec.otx = 0.0; rec.oty = 0.0; rec.otz = 0.0;
+            rec.mnx = 0.0; rec.mny = 0.0; rec.mnz = 0.0;
+            rec.mtx = 0.0; rec.mty = 0.0; rec.mtz = 0.0;
+            rec.invL0 = 1.0 / sqrt(rm2);
+            rec.sc = 0.0; rec.tc = 0.0; rec.pad = 0.0;
+            s.brec[qb] = rec;
             bnd |= 1u << k;
             ++nb;
         }
@@ -880,6 +910,14 @@ __global__ void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
     s.rbuild[i] = pi;
 }
 
+// Per-bond strengths drawn on the host, scattered into the bond records.
+__global__ void k_set_strengths(BondRec* brec, const double* sc, const double* tc, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    brec[q].sc = sc[q];
+    brec[q].tc = tc[q];
+}
+
 __global__ void k_clear_flags(Ctrl* ctrl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctrl->rebuild = 0;
@@ -956,6 +994,12 @@ hipError_t launch_build_candidates(const DevState& s, const DevParams& p, Ctrl* 
                                    hipStream_t st) {
     hipLaunchKernelGGL(k_build_candidates, dim3(grid_for(p.n, 128)), dim3(128), 0, st, s, p,
                        ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_strengths(BondRec* brec, const double* sc, const double* tc, int64_t n,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_set_strengths, dim3(grid_for(n, 256)), dim3(256), 0, st, brec, sc, tc, n);
     return hipGetLastError();
 }
 

@@ -243,22 +243,14 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     ALLOC(d.M, n);
     ALLOC(d.rbuild, n);
     ALLOC(d.nbr, nS);
-    ALLOC(d.otx, nS);
-    ALLOC(d.oty, nS);
-    ALLOC(d.otz, nS);
+    ALLOC(d.ot, nS);
     ALLOC(d.occ, n);
     ALLOC(d.bnd, n);
     ALLOC(d.otnz, n);
     ALLOC(d.nbond, n);
     ALLOC(d.bslot, nB);
     ALLOC(d.bpart, nB);
-    ALLOC(d.binvL0, nB);
-    ALLOC(d.bsc, nB);
-    ALLOC(d.btc, nB);
-    ALLOC(d.bpnx, nB); ALLOC(d.bpny, nB); ALLOC(d.bpnz, nB);
-    ALLOC(d.botx, nB); ALLOC(d.boty, nB); ALLOC(d.botz, nB);
-    ALLOC(d.bmnx, nB); ALLOC(d.bmny, nB); ALLOC(d.bmnz, nB);
-    ALLOC(d.bmtx, nB); ALLOC(d.bmty, nB); ALLOC(d.bmtz, nB);
+    ALLOC(d.brec, nB);
     ALLOC(d.cand, nC);
     ALLOC(d.cslot, nC);
     ALLOC(d.ncand, n);
@@ -419,8 +411,18 @@ rf_status rf_build_bonds(rf_ctx* c) {
             tc[i * B + b] = bond_strength(c->mat.strength_seed, lo, hi, 1, c->mat.tau_c,
                                           c->mat.tau_c_cv);
         }
-    HIP_TRY(hipMemcpy(d.bsc, sc.data(), n * B * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.btc, tc.data(), n * B * sizeof(double), hipMemcpyHostToDevice));
+    {
+        double* dsc = nullptr;
+        double* dtc = nullptr;
+        HIP_TRY(hipMalloc(&dsc, n * B * sizeof(double)));
+        HIP_TRY(hipMalloc(&dtc, n * B * sizeof(double)));
+        HIP_TRY(hipMemcpy(dsc, sc.data(), n * B * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dtc, tc.data(), n * B * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(rf::launch_set_strengths(d.brec, dsc, dtc, n * B, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipFree(dsc));
+        HIP_TRY(hipFree(dtc));
+    }
     HIP_TRY(rf::launch_prune_initial(d, c->p, c->ctrl, c->stream));
     st = rebuild(c);
     if (st) return st;
@@ -750,25 +752,18 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
     split(v->R, hp); split(v->V, hv); split(v->W, hw); split(v->F, hf); split(v->M, hm);
     std::vector<int32_t> nbr(n * S), nbond(n), bpart(n * B);
     std::vector<uint32_t> occ(n), bnd(n), otnz(n);
-    std::vector<double> otx(n * S), oty(n * S), otz(n * S);
+    std::vector<double4> ot(n * S);
     std::vector<int8_t> bslot(n * B);
     HIP_TRY(hipMemcpy(nbr.data(), d.nbr, n * S * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(occ.data(), d.occ, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bnd.data(), d.bnd, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(otnz.data(), d.otnz, n * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(otx.data(), d.otx, n * S * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(oty.data(), d.oty, n * S * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(otz.data(), d.otz, n * S * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ot.data(), d.ot, n * S * sizeof(double4), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(nbond.data(), d.nbond, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bslot.data(), d.bslot, n * B, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bpart.data(), d.bpart, n * B * 4, hipMemcpyDeviceToHost));
-    std::vector<double> bf[15];
-    double* bsrc[15] = {d.binvL0, d.bsc, d.btc, d.bpnx, d.bpny, d.bpnz, d.botx, d.boty, d.botz,
-                        d.bmnx, d.bmny, d.bmnz, d.bmtx, d.bmty, d.bmtz};
-    for (int f = 0; f < 15; ++f) {
-        bf[f].resize(n * B);
-        HIP_TRY(hipMemcpy(bf[f].data(), bsrc[f], n * B * 8, hipMemcpyDeviceToHost));
-    }
+    std::vector<rf::BondRec> brec(n * B);
+    HIP_TRY(hipMemcpy(brec.data(), d.brec, n * B * sizeof(rf::BondRec), hipMemcpyDeviceToHost));
     // The device prunes a slot at the start of the next step; the reference
     // prunes it at the end of this one (d_DeleteFarLinks).  Report the state
     // after that end-of-step prune, with the kernel's predicate.
@@ -795,9 +790,9 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
             if (v->bidx) v->bidx[q] = -1;
             const bool onz = oc && !bd && ((otnz[i] >> k) & 1u);
             if (v->o_t) {
-                v->o_t[3 * q] = onz ? otx[q] : 0.0;
-                v->o_t[3 * q + 1] = onz ? oty[q] : 0.0;
-                v->o_t[3 * q + 2] = onz ? otz[q] : 0.0;
+                v->o_t[3 * q] = onz ? ot[q].x : 0.0;
+                v->o_t[3 * q + 1] = onz ? ot[q].y : 0.0;
+                v->o_t[3 * q + 2] = onz ? ot[q].z : 0.0;
             }
             if (v->inv_L0) v->inv_L0[q] = 0.0;
             if (v->sigma_c) v->sigma_c[q] = 0.0;
@@ -813,15 +808,18 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
             const int ks = bslot[qb];
             if (ks < 0 || !((bnd[i] >> ks) & 1u) || !((occ[i] >> ks) & 1u)) continue;
             const int64_t q = i * S + ks;
+            const rf::BondRec& r = brec[qb];
             if (v->bidx) v->bidx[q] = (int8_t)b;
-            if (v->inv_L0) v->inv_L0[q] = bf[0][qb];
-            if (v->sigma_c) v->sigma_c[q] = bf[1][qb];
-            if (v->tau_c) v->tau_c[q] = bf[2][qb];
+            if (v->inv_L0) v->inv_L0[q] = r.invL0;
+            if (v->sigma_c) v->sigma_c[q] = r.sc;
+            if (v->tau_c) v->tau_c[q] = r.tc;
+            const double pn[3] = {r.pnx, r.pny, r.pnz}, o[3] = {r.otx, r.oty, r.otz};
+            const double mn[3] = {r.mnx, r.mny, r.mnz}, mt[3] = {r.mtx, r.mty, r.mtz};
             for (int cc = 0; cc < 3; ++cc) {
-                if (v->prev_n) v->prev_n[3 * q + cc] = bf[3 + cc][qb];
-                if (v->o_t) v->o_t[3 * q + cc] = bf[6 + cc][qb];
-                if (v->m_n) v->m_n[3 * q + cc] = bf[9 + cc][qb];
-                if (v->m_t) v->m_t[3 * q + cc] = bf[12 + cc][qb];
+                if (v->prev_n) v->prev_n[3 * q + cc] = pn[cc];
+                if (v->o_t) v->o_t[3 * q + cc] = o[cc];
+                if (v->m_n) v->m_n[3 * q + cc] = mn[cc];
+                if (v->m_t) v->m_t[3 * q + cc] = mt[cc];
             }
         }
     }
@@ -857,10 +855,10 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
     HIP_TRY(hipMemcpy(d.M, hm.data(), n * sizeof(double4), hipMemcpyHostToDevice));
     std::vector<int32_t> nbr(n * S, 0), nbond(n, 0), bpart(n * B, 0);
     std::vector<uint32_t> occ(n, 0), bnd(n, 0), otnz(n, 0);
-    std::vector<double> otx(n * S, 0.0), oty(n * S, 0.0), otz(n * S, 0.0);
+    std::vector<double4> ot(n * S, make_double4(0.0, 0.0, 0.0, 0.0));
     std::vector<int8_t> bslot(n * B, -1);
-    std::vector<double> bf[15];
-    for (int f = 0; f < 15; ++f) bf[f].assign(n * B, 0.0);
+    std::vector<rf::BondRec> brec(n * B);
+    std::memset(brec.data(), 0, brec.size() * sizeof(rf::BondRec));
     for (int64_t i = 0; i < n; ++i)
         for (int64_t k = 0; k < S; ++k) {
             const int64_t q = i * S + k;
@@ -876,19 +874,22 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
                 bslot[qb] = (int8_t)k;
                 bpart[qb] = v->nbr[q];
                 nbond[i] = std::max(nbond[i], b + 1);
-                bf[0][qb] = v->inv_L0 ? v->inv_L0[q] : 0.0;
-                bf[1][qb] = v->sigma_c ? v->sigma_c[q] : 0.0;
-                bf[2][qb] = v->tau_c ? v->tau_c[q] : 0.0;
+                rf::BondRec& r = brec[qb];
+                r.invL0 = v->inv_L0 ? v->inv_L0[q] : 0.0;
+                r.sc = v->sigma_c ? v->sigma_c[q] : 0.0;
+                r.tc = v->tau_c ? v->tau_c[q] : 0.0;
+                double* pn[3] = {&r.pnx, &r.pny, &r.pnz};
+                double* o[3] = {&r.otx, &r.oty, &r.otz};
+                double* mn[3] = {&r.mnx, &r.mny, &r.mnz};
+                double* mt[3] = {&r.mtx, &r.mty, &r.mtz};
                 for (int cc = 0; cc < 3; ++cc) {
-                    bf[3 + cc][qb] = v->prev_n ? v->prev_n[3 * q + cc] : 0.0;
-                    bf[6 + cc][qb] = v->o_t ? v->o_t[3 * q + cc] : 0.0;
-                    bf[9 + cc][qb] = v->m_n ? v->m_n[3 * q + cc] : 0.0;
-                    bf[12 + cc][qb] = v->m_t ? v->m_t[3 * q + cc] : 0.0;
+                    *pn[cc] = v->prev_n ? v->prev_n[3 * q + cc] : 0.0;
+                    *o[cc] = v->o_t ? v->o_t[3 * q + cc] : 0.0;
+                    *mn[cc] = v->m_n ? v->m_n[3 * q + cc] : 0.0;
+                    *mt[cc] = v->m_t ? v->m_t[3 * q + cc] : 0.0;
                 }
             } else if (v->o_t) {
-                otx[q] = v->o_t[3 * q];
-                oty[q] = v->o_t[3 * q + 1];
-                otz[q] = v->o_t[3 * q + 2];
+                ot[q] = make_double4(v->o_t[3 * q], v->o_t[3 * q + 1], v->o_t[3 * q + 2], 0.0);
                 otnz[i] |= 1u << k;
             }
         }
@@ -896,16 +897,11 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
     HIP_TRY(hipMemcpy(d.occ, occ.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.bnd, bnd.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.otnz, otnz.data(), n * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.otx, otx.data(), n * S * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.oty, oty.data(), n * S * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d.otz, otz.data(), n * S * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.ot, ot.data(), n * S * sizeof(double4), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.nbond, nbond.data(), n * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.bslot, bslot.data(), n * B, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.bpart, bpart.data(), n * B * 4, hipMemcpyHostToDevice));
-    double* bdst[15] = {d.binvL0, d.bsc, d.btc, d.bpnx, d.bpny, d.bpnz, d.botx, d.boty, d.botz,
-                        d.bmnx, d.bmny, d.bmnz, d.bmtx, d.bmty, d.bmtz};
-    for (int f = 0; f < 15; ++f)
-        HIP_TRY(hipMemcpy(bdst[f], bf[f].data(), n * B * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.brec, brec.data(), n * B * sizeof(rf::BondRec), hipMemcpyHostToDevice));
     st = rebuild(c);
     if (st) return st;
     st = sync_ctrl(c);

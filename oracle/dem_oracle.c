@@ -39,7 +39,7 @@
 #define DAMP (-1.8257)                       /* -1.8257 = -2*sqrt(5/6), md_calculateforcesdem3.cu:248 */
 #define TWO_THIRDS (2.0 / 3.0)               /* MCf_2d3 */
 #define RED_BLOCK 8                          /* particles per platen partial (one device block) */
-#define CTRL_T 256                           /* threads of the final platen-force sum */
+#define CTRL_T 1024                          /* threads of the final platen-force sum */
 #define TREE_W 32                            /* width of the per-particle contribution tree */
 
 struct ora_ctx {
@@ -883,8 +883,8 @@ static void platen_particle(ora_ctx* c, int64_t i, double* sb, double* st) {
 /* Deterministic platen-force sum, the same order as the device: per block
  * of RED_BLOCK = 8 consecutive particles a pairwise tree (v[t] += v[t+off],
  * off = 4, 2, 1; -0.0 for indices past n, as the reference's idle threads
- * store -sfb = -0.0), then CTRL_T = 256 strided running sums over the block
- * partials and a 256-wide tree.  The reference's FP32 64-thread tree and host
+ * store -sfb = -0.0), then CTRL_T = 1024 strided running sums over the block
+ * partials and a 1024-wide tree.  The reference's FP32 64-thread tree and host
  * loop (md_compression_hm.cu:243-300, md_results.cu:56-69) sum the same terms
  * in another order. */
 static double tree_n(double* v, int w) {
