@@ -8,13 +8,16 @@
 
 namespace rf {
 
-// Particles handled by one 256-thread block of the step kernel: one 32-lane
-// group per particle, eight groups per block.
-constexpr int kGroup = 32;
+// Particles handled by one 256-thread block of the step kernel: one 8-lane
+// group per particle, 32 groups per block.  Lane l of a group owns slots,
+// bonds and candidates l, l+8, l+16, l+24.
+constexpr int kLanes = 8;
 constexpr int kBlock = 256;
-constexpr int kPerBlock = kBlock / kGroup;  // 8 particles per block
-constexpr int kMaxSlots = 32;               // slots per particle (one lane each)
-constexpr int kMaxCand = 64;                // candidates per particle (two per lane)
+constexpr int kPerBlock = kBlock / kLanes;  // 32 particles per block
+constexpr int kPerLane = 4;                 // entries per lane and list
+constexpr int kMaxSlots = kLanes * kPerLane;  // 32 slots per particle
+constexpr int kMaxCand = kLanes * kPerLane;   // 32 candidates per particle
+constexpr int kMaxBonds = kLanes * kPerLane;  // 32 bonds per particle
 constexpr int kCtrlThreads = 1024;          // threads of the controller block
 
 // Error bits raised by device code (Ctrl::err).

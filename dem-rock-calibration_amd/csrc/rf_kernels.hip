@@ -43,30 +43,30 @@ constexpr double kTwoThirds = 2.0 / 3.0;   // MCf_2d3
 // side along that axis": own, x, y, z, xy, xz, yz, xyz.
 __constant__ int kStencilPos[8] = {0, 1, 2, 4, 3, 5, 6, 7};
 
-__device__ __forceinline__ uint32_t group_ballot(bool pred) {
+// Bits of an 8-lane group (bit l = lane l of the group).
+__device__ __forceinline__ uint32_t grp_ballot(bool pred) {
     const unsigned long long m = __ballot(pred);
-    const int half = (threadIdx.x >> 5) & 1;
-    return (uint32_t)(m >> (32 * half));
+    const int gw = (threadIdx.x >> 3) & 7;
+    return (uint32_t)((m >> (8 * gw)) & 0xffu);
 }
 
-__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+__device__ __forceinline__ uint32_t grp_or(uint32_t v) {
 #pragma unroll
-    for (int off = 16; off >= 1; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off, 32);
+    for (int off = kLanes / 2; off >= 1; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off, kLanes);
     return v;
 }
 
-// Fixed pairwise tree over the 32 lanes of a group; every lane gets the
-// bitwise-identical total (a + b == b + a).  Lane 0 computes
-// v[t] = v[t] + v[t + off] for off = 16..1, the CPU restatement's order.
-__device__ __forceinline__ double tree32(double v) {
+// Sum of the 8 lane partials by the tree v[t] = v[t] + v[t + off], off =
+// 4, 2, 1; every lane gets the bitwise-identical total (a + b == b + a).
+__device__ __forceinline__ double tree8(double v) {
 #pragma unroll
-    for (int off = 16; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 32);
+    for (int off = kLanes / 2; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, kLanes);
     return v;
 }
 
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-    const int lo = __shfl((int)(uint32_t)v, src, 32);
-    const int hi = __shfl((int)(uint32_t)(v >> 32), src, 32);
+    const int lo = __shfl((int)(uint32_t)v, src, kLanes);
+    const int hi = __shfl((int)(uint32_t)(v >> 32), src, kLanes);
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
@@ -86,8 +86,6 @@ __device__ __forceinline__ int free_slot(uint32_t freem, int r, bool q0) {
     }
     return r < __popc(freem) ? nth_bit(freem, r) : -1;
 }
-
-__device__ __forceinline__ double ri_of(const double4& p) { return p.w; }
 
 struct WallOut {
     double fnx, fny, fnz, ftx, fty, ftz, mtx, mty, mtz, max_, may, maz, ox, oy, oz;
@@ -149,16 +147,18 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 }
 
 // ------------------------------------------------------------ step kernel --
-// Memory round trips per group: (1) everything indexed by the particle,
-// (2) one batch of gathers with clamped indices (slot partners, candidates,
-// bond partners and bond records), (3) velocities of overlapping contacts.
+// One 8-lane group per particle; lane l owns slots, candidates and bonds
+// l + 8t (t = 0..3).  Per-particle sums: each lane adds its own entries in
+// ascending order, then tree8.  Memory round trips: rows of the particle;
+// slot-partner and candidate gathers; per bond iteration the partner's state
+// and the 128-byte bond record; the few contacts that can overlap.
 __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* __restrict__ ctrl) {
     __shared__ double sh_b[kPerBlock];
     __shared__ double sh_t[kPerBlock];
     __shared__ int sh_nb[kPerBlock];
     if (ctrl->halt) return;  // uniform for the whole grid
-    const int lane = threadIdx.x & 31;
-    const int grp = threadIdx.x >> 5;
+    const int l = threadIdx.x & (kLanes - 1);
+    const int grp = threadIdx.x / kLanes;
     const int64_t i = (int64_t)blockIdx.x * kPerBlock + grp;
     const bool valid = i < p.n;
     const int cur = ctrl->cur;
@@ -166,323 +166,335 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
     const double4* __restrict__ vel = s.vel[cur];
     const double4* __restrict__ angv = s.angv[cur];
     const double top = ctrl->top, bottom = ctrl->bottom, vw = ctrl->vw;
-    double part_b = -0.0, part_t = -0.0;
-    int bonds_after = 0;
+    const int S = p.S, C = p.C, B = p.B;
+    const int64_t ic = valid ? i : 0;  // clamped index for the inactive groups' loads
+    const double4 pi = pos[ic];
+    const double4 vi = vel[ic];
+    const double4 wi = angv[ic];
+    const double ri = pi.w, mi = vi.w;
+    uint32_t occ = valid ? s.occ[ic] : 0u;
+    uint32_t bnd = valid ? s.bnd[ic] : 0u;
+    uint32_t otnz = valid ? s.otnz[ic] : 0u;
+    const int nc = valid ? s.ncand[ic] : 0;
+    const int nbd = valid ? s.nbond[ic] : 0;
 
-    if (valid) {
-        const int S = p.S, C = p.C, B = p.B;
-        // ---- round trip 1: rows of particle i
-        const double4 pi = pos[i];
-        const double4 vi = vel[i];
-        const double4 wi = angv[i];
-        uint32_t occ = s.occ[i], bnd = s.bnd[i], otnz = s.otnz[i];
-        const int nc = s.ncand[i];
-        const int nbd = s.nbond[i];
-        const int k = lane;
-        const int64_t qs = i * S + k;
-        const int32_t jk_raw = k < S ? s.nbr[qs] : 0;
-        const int64_t qc0 = i * C + lane, qc1 = qc0 + 32;
-        const bool in0 = lane < nc, in1 = lane + 32 < nc;
-        const int32_t cj0 = in0 ? s.cand[qc0] : 0;
-        const int32_t cj1 = in1 ? s.cand[qc1] : 0;
-        const int cs0 = in0 ? (int)s.cslot[qc0] : -1;
-        const int cs1 = in1 ? (int)s.cslot[qc1] : -1;
-        const int64_t qb = i * B + lane;
-        const bool inb = lane < nbd;
-        const int32_t bj_raw = inb ? s.bpart[qb] : 0;
-        const int bks = inb ? (int)s.bslot[qb] : -1;
-
-        // ---- round trip 2: gathers (clamped to particle i where unused)
+    // ---- 1a. prune: each lane re-checks the partners of its slots (d_DeleteFarLinks)
+    int32_t jk[kPerLane];
+    uint32_t prn_l = 0u, pc_l = 0u;  // this lane's pruned / possibly-touching slots
+#pragma unroll
+    for (int t = 0; t < kPerLane; ++t) {
+        const int k = l + kLanes * t;
+        jk[t] = (k < S) ? s.nbr[ic * S + k] : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < kPerLane; ++t) {
+        const int k = l + kLanes * t;
         const bool had = k < S && ((occ >> k) & 1u);
-        const int32_t jk = had ? jk_raw : (int32_t)i;
-        const bool ab0 = in0 && cs0 < 0, ab1 = in1 && cs1 < 0;
-        const bool bl0 = inb && bks >= 0 && ((bnd >> bks) & 1u);
-        const int32_t bj = bl0 ? bj_raw : (int32_t)i;
-        const double4 pjk = pos[jk];
-        const double4 pc0 = pos[ab0 ? cj0 : (int32_t)i];
-        const double4 pc1 = pos[ab1 ? cj1 : (int32_t)i];
-        const double4 pbj = pos[bj];
-        const double4 vbj = vel[bj];
-        const double4 wbj = angv[bj];
-        const double2* rp = reinterpret_cast<const double2*>(s.brec + (bl0 ? qb : i * B));
+        const double4 pj = pos[had ? jk[t] : (int32_t)ic];
+        if (had) {
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double rr = dx * dx + dy * dy + dz * dz;
+            if (rr > p.acut2 || rr < 1e-10) prn_l |= 1u << k;
+            const double rs = ri + pj.w;
+            if (rr < rs * rs * 1.000001) pc_l |= 1u << k;
+        }
+    }
+    const uint32_t prune = grp_or(prn_l);
+    occ &= ~prune;
+    bnd &= ~prune;
+    otnz &= ~prune;
+
+    // ---- 1b. append: candidates (d_ReConstructInteractionListbpm)
+    const double fx = floor(pi.x * p.inv_cell), fy = floor(pi.y * p.inv_cell),
+                 fz = floor(pi.z * p.inv_cell);
+    const double sx = copysign(1.0, pi.x - (fx + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
+    const double sy = copysign(1.0, pi.y - (fy + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
+    const double sz = copysign(1.0, pi.z - (fz + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
+    uint64_t key[kPerLane];
+    int32_t cj[kPerLane];
+    uint32_t nw_l = 0u;   // bit t: candidate l + 8t is a new pair
+    uint32_t npc = 0u;    // bit t: the new pair may touch
+#pragma unroll
+    for (int t = 0; t < kPerLane; ++t) {
+        const int c = l + kLanes * t;
+        const bool in = c < nc && c < C;
+        const int cs = in ? (int)s.cslot[ic * C + c] : -1;
+        cj[t] = in ? s.cand[ic * C + c] : (int32_t)ic;
+        key[t] = 0;
+        if (cs >= 0 && ((prune >> cs) & 1u)) s.cslot[ic * C + c] = -1;
+        const bool ab = in && cs < 0;
+        const double4 pj = pos[ab ? cj[t] : (int32_t)ic];
+        if (ab) {
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double rr = dx * dx + dy * dy + dz * dz;
+            if (rr < p.acut2 && rr > 1e-10) {
+                const double gx = floor(pj.x * p.inv_cell), gy = floor(pj.y * p.inv_cell),
+                             gz = floor(pj.z * p.inv_cell);
+                const int bx = gx == fx ? 0 : (gx == fx + sx ? 1 : -1);
+                const int by = gy == fy ? 0 : (gy == fy + sy ? 1 : -1);
+                const int bz = gz == fz ? 0 : (gz == fz + sz ? 1 : -1);
+                if (bx >= 0 && by >= 0 && bz >= 0) {
+                    key[t] = ((uint64_t)kStencilPos[bx | (by << 1) | (bz << 2)] << 32) |
+                             (uint32_t)cj[t];
+                    nw_l |= 1u << t;
+                    const double rs = ri + pj.w;
+                    if (rr < rs * rs * 1.000001) npc |= 1u << t;
+                }
+            }
+        }
+    }
+    uint32_t nbt[kPerLane];
+    uint32_t anynew = 0u;
+#pragma unroll
+    for (int t = 0; t < kPerLane; ++t) {
+        nbt[t] = grp_ballot((nw_l >> t) & 1u);
+        anynew |= nbt[t];
+    }
+    if (anynew) {  // uniform within the group; rare
+        // Rank the new pairs by visiting order (stencil cell, then id) and
+        // give them the free slots in the reference's fill order.
+        int rank[kPerLane] = {0, 0, 0, 0};
+        int total = 0;
+#pragma unroll
+        for (int t2 = 0; t2 < kPerLane; ++t2) {
+            uint32_t m = nbt[t2];
+            total += __popc(m);
+            while (m) {
+                const int L = __builtin_ctz(m);
+                m &= m - 1u;
+                const uint64_t kb = shfl_u64(key[t2], L);
+#pragma unroll
+                for (int t = 0; t < kPerLane; ++t)
+                    if (((nw_l >> t) & 1u) && kb < key[t]) ++rank[t];
+            }
+        }
+        const uint32_t freem = ~occ & (S >= 32 ? 0xffffffffu : ((1u << S) - 1u));
+        const bool q0 = (p.quirks & RF_QUIRK_SLOT0_SENTINEL) && i == 0;
+        if (total > __popc(freem) && l == 0) atomicOr(&ctrl->err, kErrRowOverflow);
+        uint32_t add_l = 0u, pcadd_l = 0u;
+#pragma unroll
+        for (int t = 0; t < kPerLane; ++t) {
+            if (!((nw_l >> t) & 1u)) continue;
+            const int sl = free_slot(freem, rank[t], q0);
+            if (sl < 0) continue;
+            s.nbr[ic * S + sl] = cj[t];
+            s.cslot[ic * C + l + kLanes * t] = (int8_t)sl;
+            add_l |= 1u << sl;
+            if ((npc >> t) & 1u) pcadd_l |= 1u << sl;
+        }
+        const uint32_t added = grp_or(add_l);
+        pc_l |= grp_or(pcadd_l);
+        // hand each filled slot's partner to its owner lane
+#pragma unroll
+        for (int t2 = 0; t2 < kPerLane; ++t2) {
+            uint32_t m = nbt[t2];
+            while (m) {
+                const int L = __builtin_ctz(m);
+                m &= m - 1u;
+                const int rk = __shfl(rank[t2], L, kLanes);
+                const int32_t j = __shfl(cj[t2], L, kLanes);
+                const int sl = free_slot(freem, rk, q0);
+                if (sl >= 0 && (sl & (kLanes - 1)) == l) {
+                    const int tt = sl / kLanes;
+#pragma unroll
+                    for (int t = 0; t < kPerLane; ++t)
+                        if (t == tt) jk[t] = j;
+                }
+            }
+        }
+        occ |= added;
+    }
+    const uint32_t pc = grp_or(pc_l);
+
+    // ---- 2a. bonds: lane l owns bonds l + 8t (d_CalculateForcesDEM_22)
+    double bfx = 0.0, bfy = 0.0, bfz = 0.0, bmx = 0.0, bmy = 0.0, bmz = 0.0;
+    uint32_t brk_l = 0u;
+#pragma unroll 1
+    for (int t = 0; t < kPerLane; ++t) {
+        const int b = l + kLanes * t;
+        if (!__any(b < nbd)) break;  // wave-uniform
+        const bool inb = b < nbd && b < B;
+        const int64_t qb = ic * B + b;
+        const int bks = inb ? (int)s.bslot[qb] : -1;
+        const int32_t bjr = inb ? s.bpart[qb] : (int32_t)ic;
+        const bool live = bks >= 0 && ((bnd >> bks) & 1u);
+        const int32_t bj = live ? bjr : (int32_t)ic;
+        const double4 pj = pos[bj];
+        const double4 vj = vel[bj];
+        const double4 wj = angv[bj];
+        const double2* rp = reinterpret_cast<const double2*>(s.brec + (live ? qb : ic * B));
         const double2 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3], r4 = rp[4], r5 = rp[5],
                       r6 = rp[6], r7 = rp[7];
-
-        // ---- 1a. prune: slot lanes re-check their partner (d_DeleteFarLinks)
-        bool prn = false;
-        if (had) {
-            const double dx = pjk.x - pi.x, dy = pjk.y - pi.y, dz = pjk.z - pi.z;
-            const double rr = dx * dx + dy * dy + dz * dz;
-            prn = rr > p.acut2 || rr < 1e-10;
+        if (!live) continue;
+        const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+        const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
+        const double inv = 1.0 / rijm;
+        const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+        const double pnx = r0.x, pny = r0.y, pnz = r1.x;
+        const double oo0 = r1.y, oo1 = r2.x, oo2 = r2.y;
+        const double on0 = r3.x, on1 = r3.y, on2 = r4.x;
+        const double ot0 = r4.y, ot1 = r5.x, ot2 = r5.y;
+        const double lam = r6.x, bsc = r6.y, btc = r7.x;
+        const double rj = pj.w;
+        const double bb = p.b_ratio > 0.0 ? p.b_ratio * (ri < rj ? ri : rj) : p.b_abs;
+        double cefn = p.E * (rijm - 1.0 / lam) * lam;
+        double Sc = cefn;
+        cefn *= bb * bb * RF_PI;
+        const double tx = pny * nz - pnz * ny;
+        const double ty = pnz * nx - pnx * nz;
+        const double tz = pnx * ny - pny * nx;
+        const double cw = ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
+        const double px = nx * cw, py = ny * cw, pz = nz * cw;
+        const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px, M1z = -py - tz * px + ty;
+        const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px, M2z = -tz * py + px - tx;
+        const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px, M3z = ty * py + tx * px + 1.0;
+        // torsion
+        const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
+        double c1 = nx * dwx + ny * dwy + nz * dwz;
+        c1 *= p.dt * 0.5 * p.G * lam;
+        const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
+        const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
+        const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
+        const double J4 = RF_PI * bb * bb * bb * bb;
+        // bending
+        const double c2 = p.dt * 0.25 * p.E * lam;
+        const double c1b = nx * dwx + ny * dwy + nz * dwz;
+        const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
+        const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
+        const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
+        const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
+        // shear force and asymmetric moment
+        const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y), hz = 0.5 * (wj.z + wi.z);
+        const double vx = vj.x - vi.x - hy * rz + hz * ry;
+        const double vy = vj.y - vi.y - hz * rx + hx * rz;
+        const double vz = vj.z - vi.z - hx * ry + hy * rx;
+        const double cn = nx * vx + ny * vy + nz * vz;
+        const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
+        const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
+        const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
+        double cef = p.G * lam;
+        double Mc = cef;
+        cef *= bb * bb * RF_PI;
+        bfx = bfx + (nx * cefn + ox * cef);
+        bfy = bfy + (ny * cefn + oy * cef);
+        bfz = bfz + (nz * cefn + oz * cef);
+        bmx = bmx + ((mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef);
+        bmy = bmy + ((mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef);
+        bmz = bmz + ((mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef);
+        Mc *= sqrt(ox * ox + oy * oy + oz * oz);
+        Sc += sqrt(mtx * mtx + mty * mty + mtz * mtz) * 4.0 * bb;
+        Mc += sqrt(mnx * mnx + mny * mny + mnz * mnz) * 2.0 * bb;
+        if (Sc > bsc || Mc > btc) {
+            brk_l |= 1u << bks;  // the entry is dead from now on
+        } else {
+            double2* wp = reinterpret_cast<double2*>(s.brec + qb);
+            wp[0] = make_double2(nx, ny);
+            wp[1] = make_double2(nz, ox);
+            wp[2] = make_double2(oy, oz);
+            wp[3] = make_double2(mnx, mny);
+            wp[4] = make_double2(mnz, mtx);
+            wp[5] = make_double2(mty, mtz);
         }
-        const uint32_t prune = group_ballot(prn);
-        occ &= ~prune;
-        bnd &= ~prune;
-        otnz &= ~prune;
+    }
+    const double Fbx = tree8(bfx), Fby = tree8(bfy), Fbz = tree8(bfz);
+    const double Mbx = tree8(bmx), Mby = tree8(bmy), Mbz = tree8(bmz);
+    bnd &= ~grp_or(brk_l);
 
-        // ---- 1b. append: candidate lanes (d_ReConstructInteractionListbpm)
-        const double fx = floor(pi.x * p.inv_cell), fy = floor(pi.y * p.inv_cell),
-                     fz = floor(pi.z * p.inv_cell);
-        const double sx = copysign(1.0, pi.x - (fx + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
-        const double sy = copysign(1.0, pi.y - (fy + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
-        const double sz = copysign(1.0, pi.z - (fz + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
-        bool nw0 = false, nw1 = false;
-        uint64_t key0 = 0, key1 = 0;
-        if (cs0 >= 0 && ((prune >> cs0) & 1u)) s.cslot[qc0] = -1;
-        if (cs1 >= 0 && ((prune >> cs1) & 1u)) s.cslot[qc1] = -1;
+    // ---- 2b. contacts: lane l owns slots l + 8t (d_CalculateForcesDEM_21)
+    const double wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
+    const bool roll = wim > 1e-18;
+    const uint32_t cmask = occ & ~bnd;
+    double cfx = 0.0, cfy = 0.0, cfz = 0.0, cmx = 0.0, cmy = 0.0, cmz = 0.0;
+    uint32_t ovl_l = 0u;
+    if (__any((cmask & pc) != 0u)) {
+#pragma unroll 1
+        for (int t = 0; t < kPerLane; ++t) {
+            const int k = l + kLanes * t;
+            const bool chk = ((cmask & pc) >> k) & 1u;
+            int32_t j = 0;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const bool ab = h == 0 ? ab0 : ab1;
-            if (ab) {
-                const double4 pj = h == 0 ? pc0 : pc1;
-                const int32_t j = h == 0 ? cj0 : cj1;
-                const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const double rr = dx * dx + dy * dy + dz * dz;
-                if (rr < p.acut2 && rr > 1e-10) {
-                    const double gx = floor(pj.x * p.inv_cell), gy = floor(pj.y * p.inv_cell),
-                                 gz = floor(pj.z * p.inv_cell);
-                    const int bx = gx == fx ? 0 : (gx == fx + sx ? 1 : -1);
-                    const int by = gy == fy ? 0 : (gy == fy + sy ? 1 : -1);
-                    const int bz = gz == fz ? 0 : (gz == fz + sz ? 1 : -1);
-                    if (bx >= 0 && by >= 0 && bz >= 0) {
-                        const uint64_t key =
-                            ((uint64_t)kStencilPos[bx | (by << 1) | (bz << 2)] << 32) | (uint32_t)j;
-                        if (h == 0) { nw0 = true; key0 = key; }
-                        else { nw1 = true; key1 = key; }
-                    }
-                }
-            }
-        }
-        const uint32_t nb0 = group_ballot(nw0), nb1 = group_ballot(nw1);
-        uint32_t added = 0u;   // slots filled this step
-        int32_t jnew = 0;      // partner of this lane's slot if it was filled this step
-        if ((nb0 | nb1) != 0u) {
-            // Rank the new pairs by visiting order (stencil cell, then id) and
-            // give them the free slots in fill order.
-            int rank0 = 0, rank1 = 0;
-            uint32_t m = nb0;
-            while (m) {
-                const int L = __builtin_ctz(m);
-                m &= m - 1u;
-                const uint64_t kb = shfl_u64(key0, L);
-                if (nw0 && kb < key0) ++rank0;
-                if (nw1 && kb < key1) ++rank1;
-            }
-            m = nb1;
-            while (m) {
-                const int L = __builtin_ctz(m);
-                m &= m - 1u;
-                const uint64_t kb = shfl_u64(key1, L);
-                if (nw0 && kb < key0) ++rank0;
-                if (nw1 && kb < key1) ++rank1;
-            }
-            const int total = __popc(nb0) + __popc(nb1);
-            const uint32_t freem = ~occ & (S >= 32 ? 0xffffffffu : ((1u << S) - 1u));
-            const bool q0 = (p.quirks & RF_QUIRK_SLOT0_SENTINEL) && i == 0;
-            if (total > __popc(freem) && lane == 0) atomicOr(&ctrl->err, kErrRowOverflow);
-            const int sl0 = nw0 ? free_slot(freem, rank0, q0) : -1;
-            const int sl1 = nw1 ? free_slot(freem, rank1, q0) : -1;
-            if (sl0 >= 0) {
-                s.nbr[i * S + sl0] = cj0;
-                s.cslot[qc0] = (int8_t)sl0;
-            }
-            if (sl1 >= 0) {
-                s.nbr[i * S + sl1] = cj1;
-                s.cslot[qc1] = (int8_t)sl1;
-            }
-            // hand each filled slot's partner to the slot's lane
-            m = nb0;
-            while (m) {
-                const int L = __builtin_ctz(m);
-                m &= m - 1u;
-                const int sl = __shfl(sl0, L, 32);
-                const int32_t j = __shfl(cj0, L, 32);
-                if (sl >= 0) {
-                    added |= 1u << sl;
-                    if (sl == k) jnew = j;
-                }
-            }
-            m = nb1;
-            while (m) {
-                const int L = __builtin_ctz(m);
-                m &= m - 1u;
-                const int sl = __shfl(sl1, L, 32);
-                const int32_t j = __shfl(cj1, L, 32);
-                if (sl >= 0) {
-                    added |= 1u << sl;
-                    if (sl == k) jnew = j;
-                }
-            }
-            occ |= added;
-        }
-
-        // ---- 2a. bonds: one lane per bond-table entry (d_CalculateForcesDEM_22)
-        double cfx = 0.0, cfy = 0.0, cfz = 0.0, cmx = 0.0, cmy = 0.0, cmz = 0.0;
-        uint32_t brk = 0u;
-        if (bl0 && !((prune >> bks) & 1u)) {
-            const double4 pj = pbj, vj = vbj, wj = wbj;
+            for (int u = 0; u < kPerLane; ++u)
+                if (u == t) j = jk[u];
+            if (!__any(chk)) continue;
+            if (!chk) continue;
+            const int64_t qs = ic * S + k;
+            const double4 pj = pos[j];
             const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
             const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-            const double inv = 1.0 / rijm;
-            const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
-            const double pnx = r0.x, pny = r0.y, pnz = r1.x;
-            const double oo0 = r1.y, oo1 = r2.x, oo2 = r2.y;
-            const double on0 = r3.x, on1 = r3.y, on2 = r4.x;
-            const double ot0 = r4.y, ot1 = r5.x, ot2 = r5.y;
-            const double lam = r6.x, bsc = r6.y, btc = r7.x;
-            const double rj = pj.w;
-            const double b = p.b_ratio > 0.0 ? p.b_ratio * (ri_of(pi) < rj ? ri_of(pi) : rj) : p.b_abs;
-            double cefn = p.E * (rijm - 1.0 / lam) * lam;
-            double Sc = cefn;
-            cefn *= b * b * RF_PI;
-            const double tx = pny * nz - pnz * ny;
-            const double ty = pnz * nx - pnx * nz;
-            const double tz = pnx * ny - pny * nx;
-            const double cw =
-                ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
-            const double px = nx * cw, py = ny * cw, pz = nz * cw;
-            const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px,
-                         M1z = -py - tz * px + ty;
-            const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px,
-                         M2z = -tz * py + px - tx;
-            const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px,
-                         M3z = ty * py + tx * px + 1.0;
-            // torsion
-            const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
-            double c1 = nx * dwx + ny * dwy + nz * dwz;
-            c1 *= p.dt * 0.5 * p.G * lam;
-            const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
-            const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
-            const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
-            const double J4 = RF_PI * b * b * b * b;
-            // bending
-            const double c2 = p.dt * 0.25 * p.E * lam;
-            const double c1b = nx * dwx + ny * dwy + nz * dwz;
-            const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
-            const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
-            const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
-            const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
-            // shear force and asymmetric moment
-            const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y), hz = 0.5 * (wj.z + wi.z);
-            const double vx = vj.x - vi.x - hy * rz + hz * ry;
-            const double vy = vj.y - vi.y - hz * rx + hx * rz;
-            const double vz = vj.z - vi.z - hx * ry + hy * rx;
-            const double cn = nx * vx + ny * vy + nz * vz;
-            const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
-            const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
-            const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
-            double cef = p.G * lam;
-            double Mc = cef;
-            cef *= b * b * RF_PI;
-            cfx = nx * cefn + ox * cef;
-            cfy = ny * cefn + oy * cef;
-            cfz = nz * cefn + oz * cef;
-            cmx = (mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef;
-            cmy = (mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef;
-            cmz = (mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef;
-            Mc *= sqrt(ox * ox + oy * oy + oz * oz);
-            Sc += sqrt(mtx * mtx + mty * mty + mtz * mtz) * 4.0 * b;
-            Mc += sqrt(mnx * mnx + mny * mny + mnz * mnz) * 2.0 * b;
-            if (Sc > bsc || Mc > btc) {
-                brk = 1u << bks;  // the entry is dead from now on
-            } else {
-                double2* wp = reinterpret_cast<double2*>(s.brec + qb);
-                wp[0] = make_double2(nx, ny);
-                wp[1] = make_double2(nz, ox);
-                wp[2] = make_double2(oy, oz);
-                wp[3] = make_double2(mnx, mny);
-                wp[4] = make_double2(mnz, mtx);
-                wp[5] = make_double2(mty, mtz);
-            }
-        }
-        const double Fbx = tree32(cfx), Fby = tree32(cfy), Fbz = tree32(cfz);
-        const double Mbx = tree32(cmx), Mby = tree32(cmy), Mbz = tree32(cmz);
-        bnd &= ~group_or(brk);
-
-        // ---- 2b. contacts: one lane per slot (d_CalculateForcesDEM_21)
-        const double ri = pi.w, mi = vi.w;
-        const double wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
-        const bool roll = wim > 1e-18;
-        cfx = 0.0; cfy = 0.0; cfz = 0.0; cmx = 0.0; cmy = 0.0; cmz = 0.0;
-        const bool isc = k < S && ((occ >> k) & 1u) && !((bnd >> k) & 1u);
-        bool ovl = false;
-        if (isc) {
-            const bool fresh = (added >> k) & 1u;
-            const int32_t j = fresh ? jnew : jk;
-            const double4 pj = fresh ? pos[j] : pjk;
-            const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-            const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-            const double inv = 1.0 / rijm;
-            const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
             const double rj = pj.w;
             const double rsum = ri + rj;
-            if (rijm < rsum) {
-                ovl = true;
-                const double4 vj = vel[j];
-                const double4 wj = angv[j];
-                const double4 oprev = ((otnz >> k) & 1u) ? s.ot[qs] : make_double4(0.0, 0.0, 0.0, 0.0);
-                const double mj = vj.w;
-                const double Rst = ri * rj / (ri + rj);
-                const double meff = mi * mj / (mi + mj);
-                double c1 = sqrt(Rst * fabs(rsum - rijm));
-                const double Kn = 2.0 * p.hmE * c1;
-                const double Kt = 8.0 * p.hmG * c1;
-                const double wsx = rj * wj.x + ri * wi.x;
-                const double wsy = rj * wj.y + ri * wi.y;
-                const double wsz = rj * wj.z + ri * wi.z;
-                const double vx = vj.x - vi.x + (ny * wsz - nz * wsy);
-                const double vy = vj.y - vi.y + (nz * wsx - nx * wsz);
-                const double vz = vj.z - vi.z + (nx * wsy - ny * wsx);
-                const double vn = nx * vx + ny * vy + nz * vz;
-                const double fn = (rijm - rsum) * Kn * kTwoThirds;
-                const double fd = kDamp * p.pA * vn * sqrt(Kn * meff);
-                const double vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
-                const double o0 = oprev.x, o1 = oprev.y, o2 = oprev.z;
-                c1 = nx * o0 + ny * o1 + nz * o2;
-                double ox = o0 - nx * c1, oy = o1 - ny * c1, oz = o2 - nz * c1;
-                c1 = ox * ox + oy * oy + oz * oz;
-                if (c1 > 1e-18) {
-                    c1 = 1.0 / c1;
-                    c1 *= o0 * o0 + o1 * o1 + o2 * o2;
-                    c1 = sqrt(c1);
-                    ox *= c1; oy *= c1; oz *= c1;
-                }
-                ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
-                double ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
-                const double ftm = ftx * ftx + fty * fty + ftz * ftz;
-                if (ftm > p.mu * p.mu * fn * fn) {
-                    c1 = p.mu * fabs(fn) * (1.0 / sqrt(ftm));
-                    ftx *= c1; fty *= c1; ftz *= c1;
-                    c1 = 1.0 / Kt;
-                    ox = ftx * c1; oy = fty * c1; oz = ftz * c1;
-                } else {
-                    c1 = kDamp * p.pA * sqrt(Kt * meff);
-                    ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
-                }
-                s.ot[qs] = make_double4(ox, oy, oz, 0.0);
-                cfx = nx * (fn + fd) + ftx;
-                cfy = ny * (fn + fd) + fty;
-                cfz = nz * (fn + fd) + ftz;
-                const double tqx = (ny * ftz - nz * fty) * ri;
-                const double tqy = (nz * ftx - nx * ftz) * ri;
-                const double tqz = (nx * fty - ny * ftx) * ri;
-                if (roll) {
-                    const double cr = -p.mur * fabs(fn) * (1.0 / sqrt(wim));
-                    cmx = tqx + wi.x * cr; cmy = tqy + wi.y * cr; cmz = tqz + wi.z * cr;
-                } else {
-                    cmx = tqx; cmy = tqy; cmz = tqz;
-                }
+            if (!(rijm < rsum)) continue;
+            const double inv = 1.0 / rijm;
+            const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+            ovl_l |= 1u << k;
+            const double4 vj = vel[j];
+            const double4 wj = angv[j];
+            const double4 oprev = ((otnz >> k) & 1u) ? s.ot[qs] : make_double4(0.0, 0.0, 0.0, 0.0);
+            const double mj = vj.w;
+            const double Rst = ri * rj / (ri + rj);
+            const double meff = mi * mj / (mi + mj);
+            double c1 = sqrt(Rst * fabs(rsum - rijm));
+            const double Kn = 2.0 * p.hmE * c1;
+            const double Kt = 8.0 * p.hmG * c1;
+            const double wsx = rj * wj.x + ri * wi.x;
+            const double wsy = rj * wj.y + ri * wi.y;
+            const double wsz = rj * wj.z + ri * wi.z;
+            const double vx = vj.x - vi.x + (ny * wsz - nz * wsy);
+            const double vy = vj.y - vi.y + (nz * wsx - nx * wsz);
+            const double vz = vj.z - vi.z + (nx * wsy - ny * wsx);
+            const double vn = nx * vx + ny * vy + nz * vz;
+            const double fn = (rijm - rsum) * Kn * kTwoThirds;
+            const double fd = kDamp * p.pA * vn * sqrt(Kn * meff);
+            const double vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
+            const double o0 = oprev.x, o1 = oprev.y, o2 = oprev.z;
+            c1 = nx * o0 + ny * o1 + nz * o2;
+            double ox = o0 - nx * c1, oy = o1 - ny * c1, oz = o2 - nz * c1;
+            c1 = ox * ox + oy * oy + oz * oz;
+            if (c1 > 1e-18) {
+                c1 = 1.0 / c1;
+                c1 *= o0 * o0 + o1 * o1 + o2 * o2;
+                c1 = sqrt(c1);
+                ox *= c1; oy *= c1; oz *= c1;
+            }
+            ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
+            double ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
+            const double ftm = ftx * ftx + fty * fty + ftz * ftz;
+            if (ftm > p.mu * p.mu * fn * fn) {
+                c1 = p.mu * fabs(fn) * (1.0 / sqrt(ftm));
+                ftx *= c1; fty *= c1; ftz *= c1;
+                c1 = 1.0 / Kt;
+                ox = ftx * c1; oy = fty * c1; oz = ftz * c1;
+            } else {
+                c1 = kDamp * p.pA * sqrt(Kt * meff);
+                ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
+            }
+            s.ot[qs] = make_double4(ox, oy, oz, 0.0);
+            cfx = cfx + (nx * (fn + fd) + ftx);
+            cfy = cfy + (ny * (fn + fd) + fty);
+            cfz = cfz + (nz * (fn + fd) + ftz);
+            const double tqx = (ny * ftz - nz * fty) * ri;
+            const double tqy = (nz * ftx - nx * ftz) * ri;
+            const double tqz = (nx * fty - ny * ftx) * ri;
+            if (roll) {
+                const double cr = -p.mur * fabs(fn) * (1.0 / sqrt(wim));
+                cmx = cmx + (tqx + wi.x * cr);
+                cmy = cmy + (tqy + wi.y * cr);
+                cmz = cmz + (tqz + wi.z * cr);
+            } else {
+                cmx = cmx + tqx; cmy = cmy + tqy; cmz = cmz + tqz;
             }
         }
-        const uint32_t cmask = group_ballot(isc), omask = group_ballot(ovl);
-        otnz = (otnz & ~cmask) | omask;
-        const double Fcx = tree32(cfx), Fcy = tree32(cfy), Fcz = tree32(cfz);
-        double Mcx = tree32(cmx), Mcy = tree32(cmy), Mcz = tree32(cmz);
-        if ((p.quirks & RF_QUIRK_ROLL_RESET) && !roll) {
-            Mcx = 0.0; Mcy = 0.0; Mcz = 0.0;
-        }
+    }
+    otnz = (otnz & ~cmask) | grp_or(ovl_l);
+    const double Fcx = tree8(cfx), Fcy = tree8(cfy), Fcz = tree8(cfz);
+    double Mcx = tree8(cmx), Mcy = tree8(cmy), Mcz = tree8(cmz);
+    if ((p.quirks & RF_QUIRK_ROLL_RESET) && !roll) {
+        Mcx = 0.0; Mcy = 0.0; Mcz = 0.0;
+    }
 
-        // ---- 3. totals, platens, kick-drift, cut
+    // ---- 3. totals, platens, kick-drift, cut (lane 0 of the group)
+    double part_b = -0.0, part_t = -0.0;
+    if (valid && l == 0) {
         double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0;
         Fx += Fbx; Fy += Fby; Fz += Fbz;
         Mx += Mbx; My += Mby; Mz += Mbz;
@@ -526,48 +538,47 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
             part_b = -s_b;
             part_t = -s_t;
         }
-        bonds_after = __popc(bnd);
-        if (lane == 0) {
-            const double2 pk = s.pk[i];
-            const double4 rb = s.rbuild[i];
-            const double nvx = vi.x + Fx * pk.x, nvy = vi.y + Fy * pk.x, nvz = vi.z + Fz * pk.x;
-            double nrx = pi.x + nvx * p.dt, nry = pi.y + nvy * p.dt, nrz = pi.z + nvz * p.dt;
-            const double nwx_ = wi.x + Mx * pk.y, nwy_ = wi.y + My * pk.y, nwz_ = wi.z + Mz * pk.y;
-            double ovx = nvx, ovy = nvy, ovz = nvz;
-            bool out = false;
-            {
-                const double dx = nrx - p.cen[0], dy = nry - p.cen[1], dz = nrz - p.cen[2];
-                if (dx - p.half[0] > 0.0 || dx + p.half[0] < 0.0) out = true;
-                if (dy - p.half[1] > 0.0 || dy + p.half[1] < 0.0) out = true;
-                if (dz - p.half[2] > 0.0 || dz + p.half[2] < 0.0) out = true;
-            }
-            if (out) {
-                nrx = p.hid[0]; nry = p.hid[1]; nrz = p.hid[2];
-                ovx = 0.0; ovy = 0.0; ovz = 0.0;
-                Fx = 0.0; Fy = 0.0; Fz = 0.0;
-            }
-            const int nxt = cur ^ 1;
-            s.pos[nxt][i] = make_double4(nrx, nry, nrz, ri);
-            s.vel[nxt][i] = make_double4(ovx, ovy, ovz, mi);
-            s.angv[nxt][i] = make_double4(nwx_, nwy_, nwz_, 0.0);
-            if (p.store_fm) {
-                s.F[i] = make_double4(Fx, Fy, Fz, 0.0);
-                s.M[i] = make_double4(Mx, My, Mz, 0.0);
-            }
-            s.occ[i] = occ;
-            s.bnd[i] = bnd;
-            s.otnz[i] = otnz;
-            const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
-            if (out || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
+        const double2 pk = s.pk[i];
+        const double4 rb = s.rbuild[i];
+        const double nvx = vi.x + Fx * pk.x, nvy = vi.y + Fy * pk.x, nvz = vi.z + Fz * pk.x;
+        double nrx = pi.x + nvx * p.dt, nry = pi.y + nvy * p.dt, nrz = pi.z + nvz * p.dt;
+        const double nwx_ = wi.x + Mx * pk.y, nwy_ = wi.y + My * pk.y, nwz_ = wi.z + Mz * pk.y;
+        double ovx = nvx, ovy = nvy, ovz = nvz;
+        bool out = false;
+        {
+            const double dx = nrx - p.cen[0], dy = nry - p.cen[1], dz = nrz - p.cen[2];
+            if (dx - p.half[0] > 0.0 || dx + p.half[0] < 0.0) out = true;
+            if (dy - p.half[1] > 0.0 || dy + p.half[1] < 0.0) out = true;
+            if (dz - p.half[2] > 0.0 || dz + p.half[2] < 0.0) out = true;
         }
-    }
-
-    // platen partials over the block's eight particles, tree v[t] += v[t+off]
-    if (lane == 0) {
+        if (out) {
+            nrx = p.hid[0]; nry = p.hid[1]; nrz = p.hid[2];
+            ovx = 0.0; ovy = 0.0; ovz = 0.0;
+            Fx = 0.0; Fy = 0.0; Fz = 0.0;
+        }
+        const int nxt = cur ^ 1;
+        s.pos[nxt][i] = make_double4(nrx, nry, nrz, ri);
+        s.vel[nxt][i] = make_double4(ovx, ovy, ovz, mi);
+        s.angv[nxt][i] = make_double4(nwx_, nwy_, nwz_, 0.0);
+        if (p.store_fm) {
+            s.F[i] = make_double4(Fx, Fy, Fz, 0.0);
+            s.M[i] = make_double4(Mx, My, Mz, 0.0);
+        }
+        s.occ[i] = occ;
+        s.bnd[i] = bnd;
+        s.otnz[i] = otnz;
+        const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
+        if (out || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
         sh_b[grp] = part_b;
         sh_t[grp] = part_t;
-        sh_nb[grp] = bonds_after;
+        sh_nb[grp] = __popc(bnd);
+    } else if (l == 0) {
+        sh_b[grp] = -0.0;
+        sh_t[grp] = -0.0;
+        sh_nb[grp] = 0;
     }
+
+    // platen partials over the block's 32 particles, tree v[t] += v[t+off]
     __syncthreads();
     if (threadIdx.x == 0) {
         double vb[kPerBlock], vt[kPerBlock];

@@ -72,7 +72,8 @@ struct Buf {
 
 struct rf_ctx {
     int64_t n = 0;
-    int32_t S = 32, C = 64, B = 32;
+    int32_t S = 32, C = 32, B = 32;
+    double skin = 0.0;  // current candidate-list skin
     rf_material mat{};
     rf_derived der{};
     rf_box box{};
@@ -174,7 +175,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     *out = nullptr;
     rf_options o{};
     o.slots = 32;
-    o.cand_cap = 64;
+    o.cand_cap = rf::kMaxCand;
     o.skin = -1.0;
     o.device = 0;
     o.block = rf::kBlock;
@@ -192,7 +193,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     if (!(mat->acut > 0.0) || !(box->cell >= 2.0 * mat->acut * 0.999999999))
         return fail(RF_ERR_ARG, "cell edge %.6g must be >= 2*acut (acut %.6g)", box->cell,
                     mat->acut);
-    if (o.skin < 0.0) o.skin = 0.125 * mat->acut;
+    if (o.skin < 0.0) o.skin = 0.08 * mat->acut;
     if (mat->acut + o.skin > box->cell)
         return fail(RF_ERR_ARG, "acut + skin must not exceed the cell edge");
     int ndev = 0;
@@ -208,6 +209,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     c->C = o.cand_cap;
     c->B = rf::kMaxSlots;
     c->mat = *mat;
+    c->skin = o.skin;
     c->box = *box;
     c->opt = o;
     c->device = o.device;
@@ -371,14 +373,27 @@ rf_status device_error(rf_ctx* c) {
     return fail(RF_ERR_STATE, "device error flags 0x%x", e);
 }
 
-// Cell sort of the current positions and fresh candidate lists.
+// Cell sort of the current positions and fresh candidate lists.  A list
+// longer than the candidate capacity halves the skin and builds again (the
+// skin only decides how often lists are rebuilt, never a result); below
+// acut/64 the overflow is an error.
 rf_status rebuild(rf_ctx* c) {
-    HIP_TRY(rf::launch_cell_keys(c->d, c->p, c->ctrl, c->stream));
-    HIP_TRY(rf::launch_cell_sort(c->d, c->p, c->ctrl, c->stream));
-    HIP_TRY(rf::launch_build_candidates(c->d, c->p, c->ctrl, c->stream));
-    HIP_TRY(rf::launch_clear_flags(c->ctrl, c->stream));
-    c->rebuilds++;
-    return RF_OK;
+    for (;;) {
+        HIP_TRY(rf::launch_cell_keys(c->d, c->p, c->ctrl, c->stream));
+        HIP_TRY(rf::launch_cell_sort(c->d, c->p, c->ctrl, c->stream));
+        HIP_TRY(rf::launch_build_candidates(c->d, c->p, c->ctrl, c->stream));
+        HIP_TRY(rf::launch_clear_flags(c->ctrl, c->stream));
+        c->rebuilds++;
+        rf_status st = sync_ctrl(c);
+        if (st) return st;
+        if (c->hctrl.err != rf::kErrCandOverflow || c->skin < c->mat.acut / 64.0) return RF_OK;
+        c->skin *= 0.5;
+        c->p.cand_r2 = (c->mat.acut + c->skin) * (c->mat.acut + c->skin);
+        c->p.skin_half2 = 0.25 * c->skin * c->skin;
+        const int32_t zero = 0;
+        HIP_TRY(hipMemcpyAsync(&c->ctrl->err, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(rf::launch_clear_flags(c->ctrl, c->stream));
+    }
 }
 
 }  // namespace

@@ -56,7 +56,7 @@ def box_reference(space, cell):
 class Context:
     """One sample on the GPU."""
 
-    def __init__(self, xyz, r, mat, box, slots=32, cand_cap=64, skin=-1.0, device=0):
+    def __init__(self, xyz, r, mat, box, slots=32, cand_cap=32, skin=-1.0, device=0):
         self.n = int(len(r))
         self.slots = int(slots)
         self._x = np.ascontiguousarray(xyz[:, 0], dtype=np.float64)

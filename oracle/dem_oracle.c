@@ -38,9 +38,9 @@
 #define PI 3.1415926535897932384626433832795 /* MC_pi, md_math_constants.h */
 #define DAMP (-1.8257)                       /* -1.8257 = -2*sqrt(5/6), md_calculateforcesdem3.cu:248 */
 #define TWO_THIRDS (2.0 / 3.0)               /* MCf_2d3 */
-#define RED_BLOCK 8                          /* particles per platen partial (one device block) */
+#define RED_BLOCK 32                         /* particles per platen partial (one device block) */
 #define CTRL_T 1024                          /* threads of the final platen-force sum */
-#define TREE_W 32                            /* width of the per-particle contribution tree */
+#define LANES 8                              /* lanes sharing one particle on the device */
 
 struct ora_ctx {
     int64_t n;
@@ -538,14 +538,20 @@ int ora_append(ora_ctx* c) {
 
 /* -------------------------------------------------- summation ---- */
 /* Per-particle sums of slot contributions.  The reference accumulates them
- * one slot after another (fs += ... in slot order).  Here, as on the device,
- * every slot (or bond) produces one contribution vector and the 32 entries
- * (zeros for unused ones) are added by a fixed pairwise tree:
- * v[t] = v[t] + v[t + off] for off = 16, 8, 4, 2, 1, result v[0].  The tree
- * differs from the sequential sum only in rounding (checked against an exact
- * sum in the tests). */
-static double tree32(double* v) {
-    for (int off = TREE_W / 2; off >= 1; off >>= 1)
+ * one slot after another (fs += ... in slot order).  Here, as on the device
+ * where 8 lanes share a particle, entry e (a bond number or a slot) belongs
+ * to lane e % 8; each lane adds its entries in ascending order starting from
+ * 0.0, and the 8 lane sums are added by the tree v[t] = v[t] + v[t + off],
+ * off = 4, 2, 1.  Unused entries are skipped.  The order differs from the
+ * sequential sum only in rounding. */
+static double lane_sum(const double* c, const unsigned char* used, int nent) {
+    double v[LANES];
+    for (int l = 0; l < LANES; ++l) {
+        v[l] = 0.0;
+        for (int e = l; e < nent; e += LANES)
+            if (used[e]) v[l] = v[l] + c[e];
+    }
+    for (int off = LANES / 2; off >= 1; off >>= 1)
         for (int t = 0; t < off; ++t) v[t] = v[t] + v[t + off];
     return v[0];
 }
@@ -555,7 +561,7 @@ static double tree32(double* v) {
  * dd_Calculate_fijn_bpm_2, dd_Calculate_M123_bpm_2, dd_Calculate_mijn_bpm_2,
  * dd_Calculate_mijt_bpm_2, dd_Calculate_fijt_mijunsym_bpm_2 and
  * dd_CheckBreakConditionDEM_2 (md_calculateforcesdem3.cu:20-200, 427-486).
- * Bond contribution to particle i (tree position = bond number):
+ * Bond contribution to particle i (summed by bond number):
  *   F: n*(E(rho-L0)/L0 * A) + o_t*(G/L0 * A)
  *   M: (m_n*J4 + m_t*c2J) + (0.5 r x o_t)*(G/L0 * A),  c2J = dt E/(4 L0) J4 (A1)
  * Pinned by two-particle closed forms (axial, shear, torsion, bending with
@@ -568,14 +574,17 @@ static void bond_row(ora_ctx* c, int64_t i, double fs[3], double ms[3]) {
     const double xi = R[i], yi = R[n + i], zi = R[2 * n + i];
     const double wix = W[i], wiy = W[n + i], wiz = W[2 * n + i];
     const double vix = V[i], viy = V[n + i], viz = V[2 * n + i];
-    double cf[3][TREE_W], cm[3][TREE_W];
-    memset(cf, 0, sizeof cf);
-    memset(cm, 0, sizeof cm);
+    double cf[3][32], cm[3][32];
+    unsigned char used[32];
+    memset(used, 0, sizeof used);
+    int nent = 0;
     for (int64_t k = 0; k < S; ++k) {
         int64_t q = i * S + k;
         int64_t j = c->nbr[q];
         if (j < 0 || c->type[q] != 1) continue;
         int bi = c->bidx[q];
+        used[bi] = 1;
+        if (bi + 1 > nent) nent = bi + 1;
         double rx = R[j] - xi, ry = R[n + j] - yi, rz = R[2 * n + j] - zi;
         double rijm = sqrt(rx * rx + ry * ry + rz * rz);
         double inv = 1.0 / rijm;
@@ -649,8 +658,8 @@ static void bond_row(ora_ctx* c, int64_t i, double fs[3], double ms[3]) {
         }
     }
     for (int k = 0; k < 3; ++k) {
-        fs[k] = tree32(cf[k]);
-        ms[k] = tree32(cm[k]);
+        fs[k] = lane_sum(cf[k], used, nent);
+        ms[k] = lane_sum(cm[k], used, nent);
     }
 }
 
@@ -677,9 +686,9 @@ static void contact_row(ora_ctx* c, int64_t i, double fs[3], double ms[3]) {
     const double ri = c->rad[i], mi = c->mass[i];
     const double wim = wix * wix + wiy * wiy + wiz * wiz;
     const int roll = wim > 1e-18;
-    double cf[3][TREE_W], cm[3][TREE_W];
-    memset(cf, 0, sizeof cf);
-    memset(cm, 0, sizeof cm);
+    double cf[3][32], cm[3][32];
+    unsigned char used[32];
+    memset(used, 0, sizeof used);
     for (int64_t k = 0; k < S; ++k) {
         int64_t q = i * S + k;
         int64_t j = c->nbr[q];
@@ -731,6 +740,7 @@ static void contact_row(ora_ctx* c, int64_t i, double fs[3], double ms[3]) {
                     ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
                 }
                 ot[0] = ox; ot[1] = oy; ot[2] = oz;
+                used[k] = 1;
                 cf[0][k] = nx * (fn + fd) + ftx;
                 cf[1][k] = ny * (fn + fd) + fty;
                 cf[2][k] = nz * (fn + fd) + ftz;
@@ -753,8 +763,8 @@ static void contact_row(ora_ctx* c, int64_t i, double fs[3], double ms[3]) {
         }
     }
     for (int k = 0; k < 3; ++k) {
-        fs[k] = tree32(cf[k]);
-        ms[k] = tree32(cm[k]);
+        fs[k] = lane_sum(cf[k], used, (int)S);
+        ms[k] = lane_sum(cm[k], used, (int)S);
     }
     if (a2 && !roll) {
         ms[0] = 0.0; ms[1] = 0.0; ms[2] = 0.0;
@@ -881,8 +891,8 @@ static void platen_particle(ora_ctx* c, int64_t i, double* sb, double* st) {
 }
 
 /* Deterministic platen-force sum, the same order as the device: per block
- * of RED_BLOCK = 8 consecutive particles a pairwise tree (v[t] += v[t+off],
- * off = 4, 2, 1; -0.0 for indices past n, as the reference's idle threads
+ * of RED_BLOCK = 32 consecutive particles a pairwise tree (v[t] += v[t+off],
+ * off = 16..1; -0.0 for indices past n, as the reference's idle threads
  * store -sfb = -0.0), then CTRL_T = 1024 strided running sums over the block
  * partials and a 1024-wide tree.  The reference's FP32 64-thread tree and host
  * loop (md_compression_hm.cu:243-300, md_results.cu:56-69) sum the same terms

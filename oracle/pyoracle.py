@@ -197,7 +197,7 @@ class Oracle:
         lib().ora_set_state(self.h, C.byref(v))
 
     def partials(self):
-        nb = (self.n + 7) // 8
+        nb = (self.n + 31) // 32
         b = np.zeros(nb)
         t = np.zeros(nb)
         lib().ora_get_partials(self.h, _ptr(b), _ptr(t), nb)
