@@ -23,6 +23,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dem-rock-calibration_amd"))
 
 HBM_PEAK_GBPS = 8000.0
+# Every 8th step-kernel launch of the timed region is bracketed by HIP events
+# (bracketing all of them slows the whole loop by the events' own latency).
+TIMING_STRIDE = 8
+# Measured HBM traffic of one step-kernel launch (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE, scaled by a copy kernel of known size), for the default workload.
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_uniaxial_100k_pmc.md")
 
 
 def log(msg):
@@ -41,8 +47,29 @@ def step_kernel_bytes(c):
     read and written (144 B/particle), slot index + kind of each occupied
     directed slot (5 B), per directed bond 1/L0 plus prev-normal, shear,
     torsion and bending histories read and written (200 B), per overlapping
-    contact its tangential overlap read and written (48 B)."""
+    contact its tangential overlap read and written (48 B).  F and M are
+    transient inside the fused step.  The reference's per-step cell sort
+    (40 B/particle) is not part of this step: candidate lists are rebuilt
+    only when a particle has moved half the skin."""
     return 144 * c.n + 5 * c.occupied + 200 * c.bonds + 48 * c.contacts
+
+
+def measured_traffic(n):
+    """HBM bytes per step-kernel launch from the committed PMC summary, when
+    it was measured on this particle count; else None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    vals = {}
+    for ln in lines:
+        if ":" in ln and ln.split(":")[0].strip() in ("n_particles", "traffic_bytes_per_launch"):
+            k, v = ln.split(":", 1)
+            vals[k.strip()] = float(v.strip().strip("`"))
+    if vals.get("n_particles") == n and "traffic_bytes_per_launch" in vals:
+        return vals["traffic_bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(w, steps, threads):
@@ -87,9 +114,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(a sample shards by slab decomposition only in design, see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py: the DEM step runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU slab decomposition of a sample is not built)", file=sys.stderr)
         sys.exit(2)
 
     from rockfit import config
@@ -115,20 +141,20 @@ def main():
     device_sync()
     log(f"warm-up {done} steps")
 
-    ctx.set_timing(True)
+    ctx.set_timing(TIMING_STRIDE)
     device_sync()
     t1 = time.perf_counter()
     done = ctx.step(args.steps)
     device_sync()
     wall = time.perf_counter() - t1
-    kernel_ms, total_ms, steps = ctx.last_timing()
+    kernel_ms, timed, total_ms, steps = ctx.last_timing()
     c = ctx.counters()
     if done != args.steps:
         log(f"WARNING: test ended after {done} of {args.steps} timed steps")
     ms_per_step = 1e3 * wall / max(done, 1)
     value = c.n * done / wall
     bytes_step = step_kernel_bytes(c)
-    kern_avg_ms = kernel_ms / max(steps, 1)
+    kern_avg_ms = kernel_ms / max(timed, 1)
     achieved = bytes_step / (kern_avg_ms * 1e-3) / 1e9
     log(f"timed {done} steps: wall {wall:.3f}s, step kernel avg {kern_avg_ms * 1e3:.1f} us, "
         f"{bytes_step / c.n:.0f} B/particle-step, rebuilds {c.rebuilds}")
@@ -162,7 +188,7 @@ def main():
                    "contacts_per_particle": c.contacts / c.n,
                    "step_kernel_us": kern_avg_ms * 1e3, "rebuilds": c.rebuilds},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(c.n)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -34,7 +34,7 @@ struct BondRec {
     double mnx, mny, mnz;   // torsion history
     double mtx, mty, mtz;   // bending history
     double invL0, sc, tc;   // 1/L0, normal and shear strength
-    double pad;
+    double L0;              // 1.0 / invL0, formed once (the same IEEE quotient)
 };
 static_assert(sizeof(BondRec) == 128, "bond record must be one 128-byte line");
 

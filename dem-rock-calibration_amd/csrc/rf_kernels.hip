@@ -41,7 +41,10 @@ constexpr double kTwoThirds = 2.0 / 3.0;   // MCf_2d3
 // Position of a stencil cell in the reference's visiting order, indexed by
 // (dx | dy << 1 | dz << 2) where d = 1 means "the neighbour on the particle's
 // side along that axis": own, x, y, z, xy, xz, yz, xyz.
-__constant__ int kStencilPos[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+// The order is 0, 1, 2, 4, 3, 5, 6, 7: only indices 3 (xy) and 4 (z) swap.
+__device__ __forceinline__ uint32_t stencil_pos(int idx) {
+    return idx == 3 ? 4u : (idx == 4 ? 3u : (uint32_t)idx);
+}
 
 // Bits of an 8-lane group (bit l = lane l of the group).
 __device__ __forceinline__ uint32_t grp_ballot(bool pred) {
@@ -147,19 +150,219 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 }
 
 // ------------------------------------------------------------ step kernel --
+#ifndef RF_XCD_REMAP
+#define RF_XCD_REMAP 1
+#endif
+#ifndef RF_GATHER_BARRIER
+#define RF_GATHER_BARRIER 1
+#endif
+#ifndef RF_STEP_WAVES_PER_EU
+#define RF_STEP_WAVES_PER_EU 0
+#endif
+#if RF_STEP_WAVES_PER_EU > 0
+#define RF_STEP_ATTR __attribute__((amdgpu_waves_per_eu(RF_STEP_WAVES_PER_EU)))
+#else
+#define RF_STEP_ATTR
+#endif
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup w runs on XCD
+// w % 8).  Give each XCD one contiguous range of particle blocks instead, so
+// the partner gathers of a block hit the L2 of its own XCD.  A bijection on
+// [0, nblk) for any nblk.
+__device__ __forceinline__ int xcd_block(int w, int nblk) {
+#if RF_XCD_REMAP
+    constexpr int kXcd = 8;
+    const int x = w % kXcd, q = w / kXcd;
+    const int base = nblk / kXcd, extra = nblk % kXcd;
+    return x * base + (x < extra ? x : extra) + q;
+#else
+    (void)nblk;
+    return w;
+#endif
+}
+
+// One parallel bond seen from particle i (d_CalculateForcesDEM_22 and the
+// dd_*_2 helpers, md_calculateforcesdem3.cu:20-200, 427-486): rotates the
+// stored history into the current frame, increments it, and returns the
+// force and moment contributions, the new history h[0..5] (double2 pairs
+// in BondRec order) and whether the bond breaks.
+__device__ __forceinline__ bool bond_eval(const DevParams& p, const double4& pi, const double4& vi,
+                                          const double4& wi, const double4& pj, const double4& vj,
+                                          const double4& wj, const double2 (&r)[8],
+                                          double (&c)[6], double2 (&h)[6]) {
+    const double ri = pi.w, rj = pj.w;
+    const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+    const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
+    const double inv = 1.0 / rijm;
+    const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+    const double pnx = r[0].x, pny = r[0].y, pnz = r[1].x;
+    const double oo0 = r[1].y, oo1 = r[2].x, oo2 = r[2].y;
+    const double on0 = r[3].x, on1 = r[3].y, on2 = r[4].x;
+    const double ot0 = r[4].y, ot1 = r[5].x, ot2 = r[5].y;
+    const double lam = r[6].x, bsc = r[6].y, btc = r[7].x, L0 = r[7].y;
+    const double bb = p.b_ratio > 0.0 ? p.b_ratio * (ri < rj ? ri : rj) : p.b_abs;
+    double cefn = p.E * (rijm - L0) * lam;
+    double Sc = cefn;
+    cefn *= bb * bb * RF_PI;
+    const double tx = pny * nz - pnz * ny;
+    const double ty = pnz * nx - pnx * nz;
+    const double tz = pnx * ny - pny * nx;
+    const double cw = ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
+    const double px = nx * cw, py = ny * cw, pz = nz * cw;
+    const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px, M1z = -py - tz * px + ty;
+    const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px, M2z = -tz * py + px - tx;
+    const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px, M3z = ty * py + tx * px + 1.0;
+    // torsion
+    const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
+    double c1 = nx * dwx + ny * dwy + nz * dwz;
+    c1 *= p.dt * 0.5 * p.G * lam;
+    const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
+    const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
+    const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
+    const double J4 = RF_PI * bb * bb * bb * bb;
+    // bending
+    const double c2 = p.dt * 0.25 * p.E * lam;
+    const double c1b = nx * dwx + ny * dwy + nz * dwz;
+    const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
+    const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
+    const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
+    const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
+    // shear force and asymmetric moment
+    const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y), hz = 0.5 * (wj.z + wi.z);
+    const double vx = vj.x - vi.x - hy * rz + hz * ry;
+    const double vy = vj.y - vi.y - hz * rx + hx * rz;
+    const double vz = vj.z - vi.z - hx * ry + hy * rx;
+    const double cn = nx * vx + ny * vy + nz * vz;
+    const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
+    const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
+    const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
+    double cef = p.G * lam;
+    double Mc = cef;
+    cef *= bb * bb * RF_PI;
+    c[0] = nx * cefn + ox * cef;
+    c[1] = ny * cefn + oy * cef;
+    c[2] = nz * cefn + oz * cef;
+    c[3] = (mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef;
+    c[4] = (mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef;
+    c[5] = (mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef;
+    // Breakage: Sc = Sc0 + |m_t| 4b > sigma_c or Mc = (G/L0)|o_t| + |m_n| 2b > tau_c.
+    // The three norms are first taken in single precision (relative error
+    // below 2e-7); when both sums stay below their strengths by more than
+    // 1e-6 of the norm terms the exact FP64 test cannot succeed either, and
+    // is skipped.  Otherwise the exact test decides.
+    const double os2 = ox * ox + oy * oy + oz * oz;
+    const double mt2 = mtx * mtx + mty * mty + mtz * mtz;
+    const double mn2 = mnx * mnx + mny * mny + mnz * mnz;
+    const double tS = (double)__builtin_amdgcn_sqrtf((float)mt2) * 4.0 * bb;
+    const double tM1 = Mc * (double)__builtin_amdgcn_sqrtf((float)os2);
+    const double tM2 = (double)__builtin_amdgcn_sqrtf((float)mn2) * 2.0 * bb;
+    const double Se = Sc + tS, Me = tM1 + tM2;
+    const bool certain = (Se + 1e-6 * tS + 1e-14 * (fabs(Se) + bb) < bsc) &
+                         (Me + 1e-6 * Me + 1e-14 * (Me + bb) < btc);
+    bool brk = false;
+    if (!certain) {
+        Mc *= sqrt(os2);
+        Sc += sqrt(mt2) * 4.0 * bb;
+        Mc += sqrt(mn2) * 2.0 * bb;
+        brk = Sc > bsc || Mc > btc;
+    }
+    h[0] = make_double2(nx, ny);
+    h[1] = make_double2(nz, ox);
+    h[2] = make_double2(oy, oz);
+    h[3] = make_double2(mnx, mny);
+    h[4] = make_double2(mnz, mtx);
+    h[5] = make_double2(mty, mtz);
+    return brk;
+}
+
+// One Hertz-Mindlin contact seen from particle i (d_CalculateForcesDEM_21 and
+// dd_*ppHM_2, md_calculateforcesdem3.cu:202-425) for a pair known to overlap
+// (rijm < ri + rj).  Returns the force and moment contributions and the new
+// tangential overlap.
+__device__ __forceinline__ void contact_eval(const DevParams& p, const double4& pi,
+                                             const double4& vi, const double4& wi,
+                                             const double4& pj, const double4& vj,
+                                             const double4& wj, const double4& oprev, double wim,
+                                             bool roll, double (&c)[6], double4& onew) {
+    const double ri = pi.w, mi = vi.w, rj = pj.w, mj = vj.w;
+    const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
+    const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
+    const double rsum = ri + rj;
+    const double inv = 1.0 / rijm;
+    const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+    const double Rst = ri * rj / (ri + rj);
+    const double meff = mi * mj / (mi + mj);
+    double c1 = sqrt(Rst * fabs(rsum - rijm));
+    const double Kn = 2.0 * p.hmE * c1;
+    const double Kt = 8.0 * p.hmG * c1;
+    const double wsx = rj * wj.x + ri * wi.x;
+    const double wsy = rj * wj.y + ri * wi.y;
+    const double wsz = rj * wj.z + ri * wi.z;
+    const double vx = vj.x - vi.x + (ny * wsz - nz * wsy);
+    const double vy = vj.y - vi.y + (nz * wsx - nx * wsz);
+    const double vz = vj.z - vi.z + (nx * wsy - ny * wsx);
+    const double vn = nx * vx + ny * vy + nz * vz;
+    const double fn = (rijm - rsum) * Kn * kTwoThirds;
+    const double fd = kDamp * p.pA * vn * sqrt(Kn * meff);
+    const double vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
+    const double o0 = oprev.x, o1 = oprev.y, o2 = oprev.z;
+    c1 = nx * o0 + ny * o1 + nz * o2;
+    double ox = o0 - nx * c1, oy = o1 - ny * c1, oz = o2 - nz * c1;
+    c1 = ox * ox + oy * oy + oz * oz;
+    if (c1 > 1e-18) {
+        c1 = 1.0 / c1;
+        c1 *= o0 * o0 + o1 * o1 + o2 * o2;
+        c1 = sqrt(c1);
+        ox *= c1; oy *= c1; oz *= c1;
+    }
+    ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
+    double ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
+    const double ftm = ftx * ftx + fty * fty + ftz * ftz;
+    if (ftm > p.mu * p.mu * fn * fn) {
+        c1 = p.mu * fabs(fn) * (1.0 / sqrt(ftm));
+        ftx *= c1; fty *= c1; ftz *= c1;
+        c1 = 1.0 / Kt;
+        ox = ftx * c1; oy = fty * c1; oz = ftz * c1;
+    } else {
+        c1 = kDamp * p.pA * sqrt(Kt * meff);
+        ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
+    }
+    onew = make_double4(ox, oy, oz, 0.0);
+    c[0] = nx * (fn + fd) + ftx;
+    c[1] = ny * (fn + fd) + fty;
+    c[2] = nz * (fn + fd) + ftz;
+    const double tqx = (ny * ftz - nz * fty) * ri;
+    const double tqy = (nz * ftx - nx * ftz) * ri;
+    const double tqz = (nx * fty - ny * ftx) * ri;
+    if (roll) {
+        const double cr = -p.mur * fabs(fn) * (1.0 / sqrt(wim));
+        c[3] = tqx + wi.x * cr; c[4] = tqy + wi.y * cr; c[5] = tqz + wi.z * cr;
+    } else {
+        c[3] = tqx; c[4] = tqy; c[5] = tqz;
+    }
+}
+
 // One 8-lane group per particle; lane l owns slots, candidates and bonds
 // l + 8t (t = 0..3).  Per-particle sums: each lane adds its own entries in
-// ascending order, then tree8.  Memory round trips: rows of the particle;
-// slot-partner and candidate gathers; per bond iteration the partner's state
-// and the 128-byte bond record; the few contacts that can overlap.
-__global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* __restrict__ ctrl) {
+// ascending order, then tree8.
+//
+// Memory schedule: every per-particle load (state, masks, slot row,
+// candidate row, first bond index) is issued first; then every gather that
+// depends only on those (partners for the prune test, absent candidates,
+// first bond's partner and record) is issued at once with clamped indices and
+// its result masked afterwards, so a wave waits on two memory round trips
+// before any force work.  Contacts gather in rounds of one slot per lane.
+// Bonds beyond the 8th and appends are rare paths.
+__global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevParams p,
+                                                            Ctrl* __restrict__ ctrl) {
     __shared__ double sh_b[kPerBlock];
     __shared__ double sh_t[kPerBlock];
     __shared__ int sh_nb[kPerBlock];
     if (ctrl->halt) return;  // uniform for the whole grid
     const int l = threadIdx.x & (kLanes - 1);
     const int grp = threadIdx.x / kLanes;
-    const int64_t i = (int64_t)blockIdx.x * kPerBlock + grp;
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int64_t i = (int64_t)blk * kPerBlock + grp;
     const bool valid = i < p.n;
     const int cur = ctrl->cur;
     const double4* __restrict__ pos = s.pos[cur];
@@ -168,80 +371,155 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
     const double top = ctrl->top, bottom = ctrl->bottom, vw = ctrl->vw;
     const int S = p.S, C = p.C, B = p.B;
     const int64_t ic = valid ? i : 0;  // clamped index for the inactive groups' loads
+
+    // ---- stage 1: per-particle loads
     const double4 pi = pos[ic];
     const double4 vi = vel[ic];
     const double4 wi = angv[ic];
-    const double ri = pi.w, mi = vi.w;
-    uint32_t occ = valid ? s.occ[ic] : 0u;
-    uint32_t bnd = valid ? s.bnd[ic] : 0u;
-    uint32_t otnz = valid ? s.otnz[ic] : 0u;
-    const int nc = valid ? s.ncand[ic] : 0;
-    const int nbd = valid ? s.nbond[ic] : 0;
-
-    // ---- 1a. prune: each lane re-checks the partners of its slots (d_DeleteFarLinks)
-    int32_t jk[kPerLane];
-    uint32_t prn_l = 0u, pc_l = 0u;  // this lane's pruned / possibly-touching slots
+    const double2 pk = s.pk[ic];
+    const double4 rb = s.rbuild[ic];
+    const uint32_t occ_raw = s.occ[ic], bnd_raw = s.bnd[ic], otnz_raw = s.otnz[ic];
+    const int nc_raw = s.ncand[ic], nbd_raw = s.nbond[ic];
+    const uint32_t occ_in = valid ? occ_raw : 0u;
+    const uint32_t bnd_in = valid ? bnd_raw : 0u;
+    uint32_t otnz = valid ? otnz_raw : 0u;
+    const int nc = valid ? nc_raw : 0;
+    const int nbd = valid ? nbd_raw : 0;
+    int32_t jk[kPerLane], cj[kPerLane];
+    int cs[kPerLane];
 #pragma unroll
     for (int t = 0; t < kPerLane; ++t) {
         const int k = l + kLanes * t;
-        jk[t] = (k < S) ? s.nbr[ic * S + k] : 0;
+        jk[t] = s.nbr[ic * S + (k < S ? k : S - 1)];
+        const int64_t qc = ic * C + (k < C ? k : C - 1);
+        cj[t] = s.cand[qc];
+        cs[t] = s.cslot[qc];
     }
+    const int64_t qb0 = ic * B + l;
+    const int bs0 = s.bslot[qb0];
+    const int32_t bp0 = s.bpart[qb0];
+
+    // ---- stage 2: gathers (clamped to particle i where not needed)
+    double4 pp[kPerLane];
+    uint32_t cab = 0u;  // bit t: entry l + 8t is a candidate without a slot
 #pragma unroll
     for (int t = 0; t < kPerLane; ++t) {
         const int k = l + kLanes * t;
-        const bool had = k < S && ((occ >> k) & 1u);
-        const double4 pj = pos[had ? jk[t] : (int32_t)ic];
-        if (had) {
-            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const double rr = dx * dx + dy * dy + dz * dz;
-            if (rr > p.acut2 || rr < 1e-10) prn_l |= 1u << k;
-            const double rs = ri + pj.w;
-            if (rr < rs * rs * 1.000001) pc_l |= 1u << k;
-        }
+        const bool had = k < S && ((occ_in >> k) & 1u);
+        pp[t] = pos[had ? jk[t] : (int32_t)ic];
+        cab |= (uint32_t)((k < nc) & (k < C) & (cs[t] < 0)) << t;
+    }
+    // Lists are rebuilt without the current partners, so the candidates to
+    // test sit in the first entries: one gather per lane covers them.
+    int tc = cab ? __builtin_ctz(cab) : kPerLane;
+    int32_t jc = (int32_t)ic;
+#pragma unroll
+    for (int u = 0; u < kPerLane; ++u)
+        if (u == tc) jc = cj[u];
+    double4 pcj = pos[jc];
+    const bool pre0 = l < nbd && bs0 >= 0 && ((bnd_in >> bs0) & 1u);
+    const int32_t bj0 = pre0 ? bp0 : (int32_t)ic;
+    const double4 pb0 = pos[bj0];
+    const double4 vb0 = vel[bj0];
+    const double4 wb0 = angv[bj0];
+    double2 rec0[8];
+    {
+        const double2* rp = reinterpret_cast<const double2*>(s.brec + (pre0 ? qb0 : ic * B));
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rec0[u] = rp[u];
+    }
+    // keep every gather above issued before the first use of any of them
+#if RF_GATHER_BARRIER
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+
+    // ---- 1a. prune (d_DeleteFarLinks) and the possible-contact prefilter
+    const double ri = pi.w;
+    uint32_t prn_l = 0u, pc_l = 0u;
+#pragma unroll
+    for (int t = 0; t < kPerLane; ++t) {
+        const int k = l + kLanes * t;
+        const bool had = k < S && ((occ_in >> k) & 1u);
+        const double dx = pp[t].x - pi.x, dy = pp[t].y - pi.y, dz = pp[t].z - pi.z;
+        const double rr = dx * dx + dy * dy + dz * dz;
+        const double rs = ri + pp[t].w;
+        // bitwise & keeps the tests branch-free (no load sunk into a branch)
+        const bool far = (rr > p.acut2) | (rr < 1e-10);
+        const bool near = rr < rs * rs * 1.000001;
+        prn_l |= (uint32_t)(had & far) << k;
+        pc_l |= (uint32_t)(had & near) << k;
     }
     const uint32_t prune = grp_or(prn_l);
-    occ &= ~prune;
-    bnd &= ~prune;
+    uint32_t occ = occ_in & ~prune;
+    uint32_t bnd = bnd_in & ~prune;
     otnz &= ~prune;
+    int nc_out = nc;
+    if (prune) {  // uniform within the group; rare
+        // A pruned partner may come back within acut before the next list
+        // rebuild: it needs a candidate entry.  Entries that already name
+        // the slot become candidates again (their link is cleared below);
+        // the others are appended.
+        uint32_t has_l = 0u;
+#pragma unroll
+        for (int t = 0; t < kPerLane; ++t) {
+            const int c = l + kLanes * t;
+            if ((c < nc) & (c < C) & (cs[t] >= 0)) has_l |= 1u << cs[t];
+        }
+        const uint32_t addm = prune & ~grp_or(has_l);
+#pragma unroll
+        for (int t = 0; t < kPerLane; ++t) {
+            const int k = l + kLanes * t;
+            if (!((addm >> k) & 1u)) continue;
+            const int e = nc + __popc(addm & ((1u << k) - 1u));
+            if (e < C) {
+                s.cand[ic * C + e] = jk[t];
+                s.cslot[ic * C + e] = -1;
+            }
+        }
+        nc_out = nc + __popc(addm);
+        if (nc_out > C) {  // list full: rebuild the lists before the next step
+            nc_out = C;
+            if (l == 0) ctrl->rebuild = 1;
+        }
+    }
 
-    // ---- 1b. append: candidates (d_ReConstructInteractionListbpm)
+    // ---- 1b. append (d_ReConstructInteractionListbpm)
     const double fx = floor(pi.x * p.inv_cell), fy = floor(pi.y * p.inv_cell),
                  fz = floor(pi.z * p.inv_cell);
     const double sx = copysign(1.0, pi.x - (fx + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
     const double sy = copysign(1.0, pi.y - (fy + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
     const double sz = copysign(1.0, pi.z - (fz + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
-    uint64_t key[kPerLane];
-    int32_t cj[kPerLane];
-    uint32_t nw_l = 0u;   // bit t: candidate l + 8t is a new pair
-    uint32_t npc = 0u;    // bit t: the new pair may touch
+    uint64_t key[kPerLane] = {0, 0, 0, 0};
+    uint32_t nw_l = 0u;  // bit t: candidate l + 8t is a new pair
+    uint32_t npc = 0u;   // bit t: the new pair may touch
+    uint32_t crem = cab;
+#pragma unroll 1
+    for (;;) {  // one candidate per lane and round, ascending t
+        const double4 pj = pcj;
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double rr = dx * dx + dy * dy + dz * dz;
+        const double gx = floor(pj.x * p.inv_cell), gy = floor(pj.y * p.inv_cell),
+                     gz = floor(pj.z * p.inv_cell);
+        const int bx = gx == fx ? 0 : (gx == fx + sx ? 1 : -1);
+        const int by = gy == fy ? 0 : (gy == fy + sy ? 1 : -1);
+        const int bz = gz == fz ? 0 : (gz == fz + sz ? 1 : -1);
+        const bool nw = (tc < kPerLane) & (rr < p.acut2) & (rr > 1e-10) & (bx >= 0) &
+                        (by >= 0) & (bz >= 0);
+        const uint64_t sp = stencil_pos(bx | (by << 1) | (bz << 2));
+        const double rs = ri + pj.w;
 #pragma unroll
-    for (int t = 0; t < kPerLane; ++t) {
-        const int c = l + kLanes * t;
-        const bool in = c < nc && c < C;
-        const int cs = in ? (int)s.cslot[ic * C + c] : -1;
-        cj[t] = in ? s.cand[ic * C + c] : (int32_t)ic;
-        key[t] = 0;
-        if (cs >= 0 && ((prune >> cs) & 1u)) s.cslot[ic * C + c] = -1;
-        const bool ab = in && cs < 0;
-        const double4 pj = pos[ab ? cj[t] : (int32_t)ic];
-        if (ab) {
-            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const double rr = dx * dx + dy * dy + dz * dz;
-            if (rr < p.acut2 && rr > 1e-10) {
-                const double gx = floor(pj.x * p.inv_cell), gy = floor(pj.y * p.inv_cell),
-                             gz = floor(pj.z * p.inv_cell);
-                const int bx = gx == fx ? 0 : (gx == fx + sx ? 1 : -1);
-                const int by = gy == fy ? 0 : (gy == fy + sy ? 1 : -1);
-                const int bz = gz == fz ? 0 : (gz == fz + sz ? 1 : -1);
-                if (bx >= 0 && by >= 0 && bz >= 0) {
-                    key[t] = ((uint64_t)kStencilPos[bx | (by << 1) | (bz << 2)] << 32) |
-                             (uint32_t)cj[t];
-                    nw_l |= 1u << t;
-                    const double rs = ri + pj.w;
-                    if (rr < rs * rs * 1.000001) npc |= 1u << t;
-                }
-            }
-        }
+        for (int u = 0; u < kPerLane; ++u)
+            if (u == tc && nw) key[u] = (sp << 32) | (uint32_t)jc;
+        nw_l |= (uint32_t)nw << tc;
+        npc |= (uint32_t)(nw & (rr < rs * rs * 1.000001)) << tc;
+        crem &= crem - 1u;  // clears bit tc
+        if (!__any(crem != 0u)) break;
+        tc = crem ? __builtin_ctz(crem) : kPerLane;
+        jc = (int32_t)ic;
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u)
+            if (u == tc) jc = cj[u];
+        pcj = pos[jc];
     }
     uint32_t nbt[kPerLane];
     uint32_t anynew = 0u;
@@ -284,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         }
         const uint32_t added = grp_or(add_l);
         pc_l |= grp_or(pcadd_l);
-        // hand each filled slot's partner to its owner lane
+        // hand each filled slot's partner to its owner lane (slot % 8)
 #pragma unroll
         for (int t2 = 0; t2 < kPerLane; ++t2) {
             uint32_t m = nbt[t2];
@@ -304,197 +582,113 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         }
         occ |= added;
     }
-    const uint32_t pc = grp_or(pc_l);
+    const uint32_t pcm = grp_or(pc_l);
 
     // ---- 2a. bonds: lane l owns bonds l + 8t (d_CalculateForcesDEM_22)
-    double bfx = 0.0, bfy = 0.0, bfz = 0.0, bmx = 0.0, bmy = 0.0, bmz = 0.0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     uint32_t brk_l = 0u;
+    {
+        const bool live = pre0 && ((bnd >> bs0) & 1u);
+        double c[6];
+        double2 h[6];
+        const bool brk = bond_eval(p, pi, vi, wi, pb0, vb0, wb0, rec0, c, h);
+        if (live) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c[u];
+            if (brk) {
+                brk_l |= 1u << bs0;  // the entry is dead from now on
+            } else {
+                double2* wp = reinterpret_cast<double2*>(s.brec + qb0);
+#pragma unroll
+                for (int u = 0; u < 6; ++u) wp[u] = h[u];
+            }
+        }
+    }
 #pragma unroll 1
-    for (int t = 0; t < kPerLane; ++t) {
+    for (int t = 1; t < kPerLane; ++t) {  // particles with more than 8 bonds
         const int b = l + kLanes * t;
         if (!__any(b < nbd)) break;  // wave-uniform
-        const bool inb = b < nbd && b < B;
-        const int64_t qb = ic * B + b;
-        const int bks = inb ? (int)s.bslot[qb] : -1;
-        const int32_t bjr = inb ? s.bpart[qb] : (int32_t)ic;
+        const int64_t qb = ic * B + (b < B ? b : B - 1);
+        const int bks = b < nbd && b < B ? (int)s.bslot[qb] : -1;
+        const int32_t bjr = s.bpart[qb];
         const bool live = bks >= 0 && ((bnd >> bks) & 1u);
         const int32_t bj = live ? bjr : (int32_t)ic;
         const double4 pj = pos[bj];
         const double4 vj = vel[bj];
         const double4 wj = angv[bj];
+        double2 rec[8];
         const double2* rp = reinterpret_cast<const double2*>(s.brec + (live ? qb : ic * B));
-        const double2 r0 = rp[0], r1 = rp[1], r2 = rp[2], r3 = rp[3], r4 = rp[4], r5 = rp[5],
-                      r6 = rp[6], r7 = rp[7];
-        if (!live) continue;
-        const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-        const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-        const double inv = 1.0 / rijm;
-        const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
-        const double pnx = r0.x, pny = r0.y, pnz = r1.x;
-        const double oo0 = r1.y, oo1 = r2.x, oo2 = r2.y;
-        const double on0 = r3.x, on1 = r3.y, on2 = r4.x;
-        const double ot0 = r4.y, ot1 = r5.x, ot2 = r5.y;
-        const double lam = r6.x, bsc = r6.y, btc = r7.x;
-        const double rj = pj.w;
-        const double bb = p.b_ratio > 0.0 ? p.b_ratio * (ri < rj ? ri : rj) : p.b_abs;
-        double cefn = p.E * (rijm - 1.0 / lam) * lam;
-        double Sc = cefn;
-        cefn *= bb * bb * RF_PI;
-        const double tx = pny * nz - pnz * ny;
-        const double ty = pnz * nx - pnx * nz;
-        const double tz = pnx * ny - pny * nx;
-        const double cw = ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
-        const double px = nx * cw, py = ny * cw, pz = nz * cw;
-        const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px, M1z = -py - tz * px + ty;
-        const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px, M2z = -tz * py + px - tx;
-        const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px, M3z = ty * py + tx * px + 1.0;
-        // torsion
-        const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
-        double c1 = nx * dwx + ny * dwy + nz * dwz;
-        c1 *= p.dt * 0.5 * p.G * lam;
-        const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
-        const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
-        const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
-        const double J4 = RF_PI * bb * bb * bb * bb;
-        // bending
-        const double c2 = p.dt * 0.25 * p.E * lam;
-        const double c1b = nx * dwx + ny * dwy + nz * dwz;
-        const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
-        const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
-        const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
-        const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
-        // shear force and asymmetric moment
-        const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y), hz = 0.5 * (wj.z + wi.z);
-        const double vx = vj.x - vi.x - hy * rz + hz * ry;
-        const double vy = vj.y - vi.y - hz * rx + hx * rz;
-        const double vz = vj.z - vi.z - hx * ry + hy * rx;
-        const double cn = nx * vx + ny * vy + nz * vz;
-        const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
-        const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
-        const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
-        double cef = p.G * lam;
-        double Mc = cef;
-        cef *= bb * bb * RF_PI;
-        bfx = bfx + (nx * cefn + ox * cef);
-        bfy = bfy + (ny * cefn + oy * cef);
-        bfz = bfz + (nz * cefn + oz * cef);
-        bmx = bmx + ((mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef);
-        bmy = bmy + ((mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef);
-        bmz = bmz + ((mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef);
-        Mc *= sqrt(ox * ox + oy * oy + oz * oz);
-        Sc += sqrt(mtx * mtx + mty * mty + mtz * mtz) * 4.0 * bb;
-        Mc += sqrt(mnx * mnx + mny * mny + mnz * mnz) * 2.0 * bb;
-        if (Sc > bsc || Mc > btc) {
-            brk_l |= 1u << bks;  // the entry is dead from now on
-        } else {
-            double2* wp = reinterpret_cast<double2*>(s.brec + qb);
-            wp[0] = make_double2(nx, ny);
-            wp[1] = make_double2(nz, ox);
-            wp[2] = make_double2(oy, oz);
-            wp[3] = make_double2(mnx, mny);
-            wp[4] = make_double2(mnz, mtx);
-            wp[5] = make_double2(mty, mtz);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rec[u] = rp[u];
+        double c[6];
+        double2 h[6];
+        const bool brk = bond_eval(p, pi, vi, wi, pj, vj, wj, rec, c, h);
+        if (live) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c[u];
+            if (brk) {
+                brk_l |= 1u << bks;
+            } else {
+                double2* wp = reinterpret_cast<double2*>(s.brec + qb);
+#pragma unroll
+                for (int u = 0; u < 6; ++u) wp[u] = h[u];
+            }
         }
     }
-    const double Fbx = tree8(bfx), Fby = tree8(bfy), Fbz = tree8(bfz);
-    const double Mbx = tree8(bmx), Mby = tree8(bmy), Mbz = tree8(bmz);
+    const double Fbx = tree8(acc[0]), Fby = tree8(acc[1]), Fbz = tree8(acc[2]);
+    const double Mbx = tree8(acc[3]), Mby = tree8(acc[4]), Mbz = tree8(acc[5]);
     bnd &= ~grp_or(brk_l);
 
     // ---- 2b. contacts: lane l owns slots l + 8t (d_CalculateForcesDEM_21)
     const double wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
     const bool roll = wim > 1e-18;
     const uint32_t cmask = occ & ~bnd;
-    double cfx = 0.0, cfy = 0.0, cfz = 0.0, cmx = 0.0, cmy = 0.0, cmz = 0.0;
-    uint32_t ovl_l = 0u;
-    if (__any((cmask & pc) != 0u)) {
-#pragma unroll 1
-        for (int t = 0; t < kPerLane; ++t) {
-            const int k = l + kLanes * t;
-            const bool chk = ((cmask & pc) >> k) & 1u;
-            int32_t j = 0;
+    uint32_t mine = 0u;  // bit t: slot l + 8t needs the exact overlap test
 #pragma unroll
-            for (int u = 0; u < kPerLane; ++u)
-                if (u == t) j = jk[u];
-            if (!__any(chk)) continue;
-            if (!chk) continue;
-            const int64_t qs = ic * S + k;
-            const double4 pj = pos[j];
-            const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-            const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-            const double rj = pj.w;
-            const double rsum = ri + rj;
-            if (!(rijm < rsum)) continue;
-            const double inv = 1.0 / rijm;
-            const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
+    for (int t = 0; t < kPerLane; ++t)
+        if (((cmask & pcm) >> (l + kLanes * t)) & 1u) mine |= 1u << t;
+#pragma unroll
+    for (int u = 0; u < 6; ++u) acc[u] = 0.0;
+    uint32_t ovl_l = 0u;
+#pragma unroll 1
+    while (__any(mine != 0u)) {  // one slot per lane and round, ascending t
+        const int t = mine ? __builtin_ctz(mine) : kPerLane;
+        mine &= mine - 1u;
+        const bool act = t < kPerLane;
+        int32_t j = (int32_t)ic;
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u)
+            if (u == t) j = jk[u];
+        const int k = l + kLanes * (act ? t : 0);
+        const int64_t qs = ic * S + k;
+        const double4 pj = pos[j];
+        const double4 vj = vel[j];
+        const double4 wj = angv[j];
+        const double4 ot = s.ot[qs];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const bool ov = act && sqrt(dx * dx + dy * dy + dz * dz) < ri + pj.w;
+        if (ov) {
+            const double4 oprev = ((otnz >> k) & 1u) ? ot : make_double4(0.0, 0.0, 0.0, 0.0);
+            double c[6];
+            double4 onew;
+            contact_eval(p, pi, vi, wi, pj, vj, wj, oprev, wim, roll, c, onew);
+            s.ot[qs] = onew;
+#pragma unroll
+            for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c[u];
             ovl_l |= 1u << k;
-            const double4 vj = vel[j];
-            const double4 wj = angv[j];
-            const double4 oprev = ((otnz >> k) & 1u) ? s.ot[qs] : make_double4(0.0, 0.0, 0.0, 0.0);
-            const double mj = vj.w;
-            const double Rst = ri * rj / (ri + rj);
-            const double meff = mi * mj / (mi + mj);
-            double c1 = sqrt(Rst * fabs(rsum - rijm));
-            const double Kn = 2.0 * p.hmE * c1;
-            const double Kt = 8.0 * p.hmG * c1;
-            const double wsx = rj * wj.x + ri * wi.x;
-            const double wsy = rj * wj.y + ri * wi.y;
-            const double wsz = rj * wj.z + ri * wi.z;
-            const double vx = vj.x - vi.x + (ny * wsz - nz * wsy);
-            const double vy = vj.y - vi.y + (nz * wsx - nx * wsz);
-            const double vz = vj.z - vi.z + (nx * wsy - ny * wsx);
-            const double vn = nx * vx + ny * vy + nz * vz;
-            const double fn = (rijm - rsum) * Kn * kTwoThirds;
-            const double fd = kDamp * p.pA * vn * sqrt(Kn * meff);
-            const double vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
-            const double o0 = oprev.x, o1 = oprev.y, o2 = oprev.z;
-            c1 = nx * o0 + ny * o1 + nz * o2;
-            double ox = o0 - nx * c1, oy = o1 - ny * c1, oz = o2 - nz * c1;
-            c1 = ox * ox + oy * oy + oz * oz;
-            if (c1 > 1e-18) {
-                c1 = 1.0 / c1;
-                c1 *= o0 * o0 + o1 * o1 + o2 * o2;
-                c1 = sqrt(c1);
-                ox *= c1; oy *= c1; oz *= c1;
-            }
-            ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
-            double ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
-            const double ftm = ftx * ftx + fty * fty + ftz * ftz;
-            if (ftm > p.mu * p.mu * fn * fn) {
-                c1 = p.mu * fabs(fn) * (1.0 / sqrt(ftm));
-                ftx *= c1; fty *= c1; ftz *= c1;
-                c1 = 1.0 / Kt;
-                ox = ftx * c1; oy = fty * c1; oz = ftz * c1;
-            } else {
-                c1 = kDamp * p.pA * sqrt(Kt * meff);
-                ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
-            }
-            s.ot[qs] = make_double4(ox, oy, oz, 0.0);
-            cfx = cfx + (nx * (fn + fd) + ftx);
-            cfy = cfy + (ny * (fn + fd) + fty);
-            cfz = cfz + (nz * (fn + fd) + ftz);
-            const double tqx = (ny * ftz - nz * fty) * ri;
-            const double tqy = (nz * ftx - nx * ftz) * ri;
-            const double tqz = (nx * fty - ny * ftx) * ri;
-            if (roll) {
-                const double cr = -p.mur * fabs(fn) * (1.0 / sqrt(wim));
-                cmx = cmx + (tqx + wi.x * cr);
-                cmy = cmy + (tqy + wi.y * cr);
-                cmz = cmz + (tqz + wi.z * cr);
-            } else {
-                cmx = cmx + tqx; cmy = cmy + tqy; cmz = cmz + tqz;
-            }
         }
     }
     otnz = (otnz & ~cmask) | grp_or(ovl_l);
-    const double Fcx = tree8(cfx), Fcy = tree8(cfy), Fcz = tree8(cfz);
-    double Mcx = tree8(cmx), Mcy = tree8(cmy), Mcz = tree8(cmz);
+    const double Fcx = tree8(acc[0]), Fcy = tree8(acc[1]), Fcz = tree8(acc[2]);
+    double Mcx = tree8(acc[3]), Mcy = tree8(acc[4]), Mcz = tree8(acc[5]);
     if ((p.quirks & RF_QUIRK_ROLL_RESET) && !roll) {
         Mcx = 0.0; Mcy = 0.0; Mcz = 0.0;
     }
 
     // ---- 3. totals, platens, kick-drift, cut (lane 0 of the group)
-    double part_b = -0.0, part_t = -0.0;
     if (valid && l == 0) {
+        const double mi = vi.w;
+        double part_b = -0.0, part_t = -0.0;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0;
         Fx += Fbx; Fy += Fby; Fz += Fbz;
         Mx += Mbx; My += Mby; Mz += Mbz;
@@ -538,8 +732,6 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
             part_b = -s_b;
             part_t = -s_t;
         }
-        const double2 pk = s.pk[i];
-        const double4 rb = s.rbuild[i];
         const double nvx = vi.x + Fx * pk.x, nvy = vi.y + Fy * pk.x, nvz = vi.z + Fz * pk.x;
         double nrx = pi.x + nvx * p.dt, nry = pi.y + nvy * p.dt, nrz = pi.z + nvz * p.dt;
         const double nwx_ = wi.x + Mx * pk.y, nwy_ = wi.y + My * pk.y, nwz_ = wi.z + Mz * pk.y;
@@ -567,6 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         s.occ[i] = occ;
         s.bnd[i] = bnd;
         s.otnz[i] = otnz;
+        if (nc_out != nc) s.ncand[i] = nc_out;
         const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
         if (out || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
         sh_b[grp] = part_b;
@@ -576,6 +769,12 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
         sh_b[grp] = -0.0;
         sh_t[grp] = -0.0;
         sh_nb[grp] = 0;
+    }
+    // candidates whose slot was pruned are absent again
+#pragma unroll
+    for (int t = 0; t < kPerLane; ++t) {
+        const int c = l + kLanes * t;
+        if (c < nc && c < C && cs[t] >= 0 && ((prune >> cs[t]) & 1u)) s.cslot[ic * C + c] = -1;
     }
 
     // platen partials over the block's 32 particles, tree v[t] += v[t+off]
@@ -596,9 +795,9 @@ __global__ __launch_bounds__(kBlock) void k_step(DevState s, DevParams p, Ctrl* 
                 vb[t] = vb[t] + vb[t + off];
                 vt[t] = vt[t] + vt[t + off];
             }
-        s.part_b[blockIdx.x] = vb[0];
-        s.part_t[blockIdx.x] = vt[0];
-        s.part_bonds[blockIdx.x] = nb;
+        s.part_b[blk] = vb[0];
+        s.part_t[blk] = vt[0];
+        s.part_bonds[blk] = nb;
     }
 }
 
@@ -836,7 +1035,7 @@ __global__ void k_construct_bonds(DevState s, DevParams p, Ctrl* ctrl) {
             rec.mnx = 0.0; rec.mny = 0.0; rec.mnz = 0.0;
             rec.mtx = 0.0; rec.mty = 0.0; rec.mtz = 0.0;
             rec.invL0 = 1.0 / sqrt(rm2);
-            rec.sc = 0.0; rec.tc = 0.0; rec.pad = 0.0;
+            rec.sc = 0.0; rec.tc = 0.0; rec.L0 = 1.0 / rec.invL0;
             s.brec[qb] = rec;
             bnd |= 1u << k;
             ++nb;
@@ -877,6 +1076,7 @@ __global__ void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
     const int64_t key = s.ckey[i];
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t C = p.C;
+    const uint32_t occ = s.occ[i];
     int cnt = 0;
     bool over = false;
     for (int dz = -1; dz <= 1; ++dz)
@@ -889,33 +1089,20 @@ __global__ void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
                     if (j == (int32_t)i) continue;
                     const double4 pj = pos[j];
                     const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-                    if (ex * ex + ey * ey + ez * ez < p.cand_r2) {
-                        if (cnt >= C) {
-                            over = true;
-                            continue;
-                        }
-                        s.cand[i * C + cnt] = j;
-                        s.cslot[i * C + cnt] = -1;
-                        ++cnt;
+                    if (!(ex * ex + ey * ey + ez * ez < p.cand_r2)) continue;
+                    bool partner = false;  // current partners hold a slot already
+                    for (uint32_t m = occ; m && !partner; m &= m - 1u)
+                        partner = s.nbr[i * p.S + __builtin_ctz(m)] == j;
+                    if (partner) continue;
+                    if (cnt >= C) {
+                        over = true;
+                        continue;
                     }
+                    s.cand[i * C + cnt] = j;
+                    s.cslot[i * C + cnt] = -1;
+                    ++cnt;
                 }
             }
-    const uint32_t occ = s.occ[i];
-    for (int k = 0; k < p.S; ++k) {
-        if (!((occ >> k) & 1u)) continue;
-        const int32_t j = s.nbr[i * p.S + k];
-        int c = 0;
-        while (c < cnt && s.cand[i * C + c] != j) ++c;
-        if (c == cnt) {
-            if (cnt >= C) {
-                over = true;
-                continue;
-            }
-            s.cand[i * C + cnt] = j;
-            ++cnt;
-        }
-        s.cslot[i * C + c] = (int8_t)k;
-    }
     if (over) atomicOr(&ctrl->err, kErrCandOverflow);
     s.ncand[i] = cnt;
     s.rbuild[i] = pi;

@@ -91,10 +91,10 @@ struct rf_ctx {
     int64_t* dcount = nullptr;
     std::vector<void*> allocs;
     int32_t chunk = 256;
-    bool timing = false;
+    int32_t timing = 0;  // bracket every timing-th step launch with events (0: off)
     std::vector<hipEvent_t> ev;
     double last_kernel_ms = 0.0, last_total_ms = 0.0;
-    int64_t last_steps = 0;
+    int64_t last_steps = 0, last_timed = 0;
     double ld = 0.0, destruction = 0.0;
 
     template <class T>
@@ -551,11 +551,18 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
     if (st) return st;
     const int64_t start = c->hctrl.step;
     int64_t remaining = nsteps;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
-    HIP_TRY(hipEventRecord(t0, c->stream));
+    struct EventPair {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~EventPair() {
+            if (a) hipEventDestroy(a);
+            if (b) hipEventDestroy(b);
+        }
+    } wall;
+    HIP_TRY(hipEventCreate(&wall.a));
+    HIP_TRY(hipEventCreate(&wall.b));
+    HIP_TRY(hipEventRecord(wall.a, c->stream));
     double kernel_ms = 0.0;
+    int64_t timed = 0;
     while (remaining > 0 && !c->hctrl.done) {
         const int64_t k = std::min<int64_t>(remaining, c->chunk);
         if (c->timing && (int64_t)c->ev.size() < 2 * k) {
@@ -567,9 +574,10 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         }
         const int64_t before = c->hctrl.step;
         for (int64_t s = 0; s < k; ++s) {
-            if (c->timing) HIP_TRY(hipEventRecord(c->ev[2 * s], c->stream));
+            const bool tm = c->timing && (before + s) % c->timing == 0;
+            if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s], c->stream));
             HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
-            if (c->timing) HIP_TRY(hipEventRecord(c->ev[2 * s + 1], c->stream));
+            if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s + 1], c->stream));
             HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
         }
         st = sync_ctrl(c);
@@ -577,42 +585,38 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         const int64_t executed = c->hctrl.step - before;
         if (c->timing)
             for (int64_t s = 0; s < executed; ++s) {
+                if ((before + s) % c->timing) continue;
                 float ms = 0.f;
                 HIP_TRY(hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]));
                 kernel_ms += ms;
+                ++timed;
             }
         remaining -= executed;
-        if (c->hctrl.err) {
-            hipEventDestroy(t0);
-            hipEventDestroy(t1);
-            return device_error(c);
-        }
+        if (c->hctrl.err) return device_error(c);
         if (c->hctrl.rebuild) {
             st = rebuild(c);
-            if (st) return st;
-            st = sync_ctrl(c);
             if (st) return st;
             if (c->hctrl.err) return device_error(c);
         } else if (executed == 0 && !c->hctrl.done) {
             return fail(RF_ERR_STATE, "loading loop halted without a reason");
         }
     }
-    HIP_TRY(hipEventRecord(t1, c->stream));
-    HIP_TRY(hipEventSynchronize(t1));
+    HIP_TRY(hipEventRecord(wall.b, c->stream));
+    HIP_TRY(hipEventSynchronize(wall.b));
     float total = 0.f;
-    HIP_TRY(hipEventElapsedTime(&total, t0, t1));
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    HIP_TRY(hipEventElapsedTime(&total, wall.a, wall.b));
     c->last_total_ms = total;
     c->last_kernel_ms = kernel_ms;
+    c->last_timed = timed;
     c->last_steps = c->hctrl.step - start;
     if (done_steps) *done_steps = c->last_steps;
     return RF_OK;
 }
 
-rf_status rf_last_timing(rf_ctx* c, double* kms, double* tms, int64_t* steps) {
+rf_status rf_last_timing(rf_ctx* c, double* kms, int64_t* timed, double* tms, int64_t* steps) {
     if (!c) return fail(RF_ERR_ARG, "null context");
     if (kms) *kms = c->last_kernel_ms;
+    if (timed) *timed = c->last_timed;
     if (tms) *tms = c->last_total_ms;
     if (steps) *steps = c->last_steps;
     return RF_OK;
@@ -891,6 +895,7 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
                 nbond[i] = std::max(nbond[i], b + 1);
                 rf::BondRec& r = brec[qb];
                 r.invL0 = v->inv_L0 ? v->inv_L0[q] : 0.0;
+                r.L0 = 1.0 / r.invL0;
                 r.sc = v->sigma_c ? v->sigma_c[q] : 0.0;
                 r.tc = v->tau_c ? v->tau_c[q] : 0.0;
                 double* pn[3] = {&r.pnx, &r.pny, &r.pnz};

@@ -1,4 +1,4 @@
-"""Loader of the HIP library librockfit_dem.so (built in-tree by build.py).
+"""Loader of the HIP library librockfit_dem.so (built in-tree by build_lib.py).
 
 There is no CPU fallback: if the library is missing or no GPU is present,
 calls raise.
@@ -9,7 +9,8 @@ import os
 from . import abi
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "librockfit_dem.so")
+# ROCKFIT_LIB names another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("ROCKFIT_LIB") or os.path.join(PKG_DIR, "lib", "librockfit_dem.so")
 
 _lib = None
 
@@ -77,7 +78,7 @@ def lib():
     L.rf_num_slots.restype = C.c_int32
     L.rf_set_timing.argtypes = [vp, C.c_int32]
     L.rf_set_timing.restype = st
-    L.rf_last_timing.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_int64)]
+    L.rf_last_timing.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_double), P(C.c_int64)]
     L.rf_last_timing.restype = st
     L.rf_set_chunk.argtypes = [vp, C.c_int32]
     L.rf_set_chunk.restype = st

@@ -139,13 +139,17 @@ class Context:
         v = state_view(st)
         check(lib().rf_set_state(self.h, C.byref(v)), "rf_set_state")
 
-    def set_timing(self, on=True):
-        check(lib().rf_set_timing(self.h, 1 if on else 0), "rf_set_timing")
+    def set_timing(self, every=1):
+        """Bracket every `every`-th step-kernel launch with HIP events (0: off)."""
+        check(lib().rf_set_timing(self.h, int(every)), "rf_set_timing")
 
     def set_chunk(self, steps):
         check(lib().rf_set_chunk(self.h, int(steps)), "rf_set_chunk")
 
     def last_timing(self):
-        k, t, s = C.c_double(), C.c_double(), C.c_int64()
-        check(lib().rf_last_timing(self.h, C.byref(k), C.byref(t), C.byref(s)), "rf_last_timing")
-        return k.value, t.value, s.value
+        """(summed ms of the bracketed step kernels, bracketed launches,
+        wall ms of the last step() call, steps it executed)"""
+        k, n, t, s = C.c_double(), C.c_int64(), C.c_double(), C.c_int64()
+        check(lib().rf_last_timing(self.h, C.byref(k), C.byref(n), C.byref(t), C.byref(s)),
+              "rf_last_timing")
+        return k.value, n.value, t.value, s.value

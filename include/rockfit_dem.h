@@ -97,8 +97,9 @@ typedef struct rf_sample {
 
 typedef struct rf_options {
     int32_t slots;        /* interaction slots per particle (IonP), default 32  */
-    int32_t cand_cap;     /* neighbour-candidate capacity per particle (GPU)   */
-    double skin;          /* candidate-list skin width (GPU); never changes results */
+    int32_t cand_cap;     /* neighbour-candidate capacity per particle, <= 32 (GPU) */
+    double skin;          /* candidate-list skin width (GPU, default 0.08 acut; halved
+                             when a list overflows); never changes results */
     int32_t device;       /* HIP device ordinal                                */
     int32_t block;        /* threads per block of the per-particle kernels     */
 } rf_options;
@@ -223,12 +224,15 @@ rf_status rf_set_state(rf_ctx* ctx, const rf_state_view* in);
 int64_t rf_num_particles(const rf_ctx* ctx);
 int32_t rf_num_slots(const rf_ctx* ctx);
 
-/* Timing: with timing enabled, rf_step brackets every launch of the fused
- * step kernel with HIP events on the context's stream.  rf_last_timing
- * returns the summed kernel time, the wall time of the last rf_step call
- * (both ms) and the number of steps it executed. */
-rf_status rf_set_timing(rf_ctx* ctx, int32_t enable);
-rf_status rf_last_timing(rf_ctx* ctx, double* step_kernel_ms, double* total_ms, int64_t* steps);
+/* Timing: rf_set_timing(ctx, k) with k > 0 brackets every k-th launch of
+ * the fused step kernel (global step index divisible by k) with HIP events
+ * on the context's stream; 0 switches it off.  rf_last_timing returns, for
+ * the last rf_step call, the summed time of the bracketed launches (ms), how
+ * many launches were bracketed, the wall time of the call (ms) and the
+ * number of steps it executed.  No counterpart in the reference. */
+rf_status rf_set_timing(rf_ctx* ctx, int32_t every);
+rf_status rf_last_timing(rf_ctx* ctx, double* step_kernel_ms, int64_t* timed_launches,
+                         double* total_ms, int64_t* steps);
 
 /* Steps per host synchronisation inside rf_step (default 256). */
 rf_status rf_set_chunk(rf_ctx* ctx, int32_t steps_per_sync);
