@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/rockfit_dem.h"
+
 namespace rf {
 
 // Particles handled by one 256-thread block of the step kernel: one 8-lane
@@ -104,6 +106,10 @@ struct Ctrl {
     int32_t loading, fracture, done, halt;
     int32_t cur, rebuild, err, pad;
 };
+
+// Sets the text rf_last_error() returns and passes `st` through
+// (rf_runtime.hip).
+rf_status report(rf_status st, const char* fmt, ...);
 
 // Launch wrappers (rf_kernels.hip).
 hipError_t launch_cell_keys(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);

@@ -19,11 +19,11 @@
 #include "rf_internal.h"
 
 namespace {
-
-constexpr double kPi = 3.1415926535897932384626433832795;  // MC_pi
 thread_local std::string g_err;
+}  // namespace
 
-rf_status fail(rf_status st, const char* fmt, ...) {
+namespace rf {
+rf_status report(rf_status st, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -32,6 +32,12 @@ rf_status fail(rf_status st, const char* fmt, ...) {
     g_err = buf;
     return st;
 }
+}  // namespace rf
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932384626433832795;  // MC_pi
+rf_status (*const fail)(rf_status, const char*, ...) = rf::report;
 
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \

@@ -26,6 +26,7 @@ EXPORTS = [
     "rf_build_bonds", "rf_set_test", "rf_step", "rf_run", "rf_get_test_state", "rf_get_curve",
     "rf_get_bond_counts", "rf_get_counters", "rf_get_state", "rf_set_state",
     "rf_num_particles", "rf_num_slots", "rf_set_timing", "rf_last_timing", "rf_set_chunk",
+    "rf_fire_defaults", "rf_fire_relax",
 ]
 
 
@@ -80,6 +81,11 @@ def lib():
     L.rf_set_timing.restype = st
     L.rf_last_timing.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_double), P(C.c_int64)]
     L.rf_last_timing.restype = st
+    L.rf_fire_defaults.argtypes = [P(abi.FireCfg)]
+    L.rf_fire_defaults.restype = st
+    L.rf_fire_relax.argtypes = [C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
+                                P(C.c_double), P(abi.FireCfg), P(C.c_double), P(abi.FireResult)]
+    L.rf_fire_relax.restype = st
     L.rf_set_chunk.argtypes = [vp, C.c_int32]
     L.rf_set_chunk.restype = st
     _lib = L

@@ -109,6 +109,34 @@ class StateView(C.Structure):
     ]
 
 
+RF_QUIRK_FIRE_NO_INERTIA = 0x8
+RF_CONTAINER_NONE = 0
+RF_CONTAINER_CYLINDER_Z = 1
+
+
+class FireCfg(C.Structure):
+    _fields_ = [
+        ("D", C.c_double), ("dt0", C.c_double), ("density", C.c_double), ("alpha0", C.c_double),
+        ("dtgrow", C.c_double), ("dtshrink", C.c_double), ("alpha_shrink", C.c_double),
+        ("dtmax_factor", C.c_double), ("dtmin_factor", C.c_double),
+        ("ndelay", C.c_int32), ("container", C.c_int32),
+        ("np_negative_max", C.c_int64), ("max_steps", C.c_int64), ("hard_cap", C.c_int64),
+        ("center", D3), ("radius", C.c_double), ("half_height", C.c_double),
+        ("hide_z", C.c_double), ("hidden", D3), ("gap_tol", C.c_double),
+        ("check_every", C.c_int32), ("delete_after", C.c_int32), ("skin", C.c_double),
+        ("quirks", C.c_uint32), ("device", C.c_int32),
+    ]
+
+
+class FireResult(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("np_positive", C.c_int64), ("np_negative", C.c_int64),
+        ("removed", C.c_int64), ("dt", C.c_double), ("alpha", C.c_double),
+        ("min_gap", C.c_double), ("power", C.c_double), ("stop_reason", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
 def struct_to_dict(s):
     out = {}
     for name, _ in s._fields_:

@@ -237,6 +237,71 @@ rf_status rf_last_timing(rf_ctx* ctx, double* step_kernel_ms, int64_t* timed_lau
 /* Steps per host synchronisation inside rf_step (default 256). */
 rf_status rf_set_chunk(rf_ctx* ctx, int32_t steps_per_sync);
 
+/* ------------------------------------------------------------------------
+ * FIRE relaxation of a packing (sample preparation).
+ *
+ * Replaces the relaxation loop of h_CreateCylinderSample
+ * (md_createcylindersample.cu:353-420) with its parts SetFIREData
+ * (md_firerelaxation.cu:72-86), d_CalculateForces (md_calculateforces.cu:20-67),
+ * d_FdotVEntire (md_firerelaxation.cu:375+), d_CalculateDecrementsHalfStepFIRE
+ * (:58-70), d_CalculateIncrementsFIRE (:18-56), d_CylinderRestrictionZ
+ * (md_cylinderrestriction.cu:69-113) and d_ReduceParticleNumber
+ * (md_reduceparticlenumber.cu:18-39).  Pair force for rho^2 < a^2,
+ * a = r_i + r_j:  F_i += -D (a^2 / rho^2) (R_j - R_i).  The FIRE controller
+ * runs on the device; the host waits once per call.
+ * ------------------------------------------------------------------------ */
+#define RF_QUIRK_FIRE_NO_INERTIA 0x8u /* V = F dt0/m overwritten each step, alpha mixing is the identity, md_firerelaxation.cu:28-52 */
+
+typedef enum rf_container {
+    RF_CONTAINER_NONE = 0,
+    RF_CONTAINER_CYLINDER_Z = 1  /* centres kept within radius and +-half_height of center */
+} rf_container;
+
+typedef struct rf_fire_cfg {
+    double D;                 /* repulsion strength (reduced force per mm)      */
+    double dt0;               /* base step (us); dtmax = 100 dt0, dtmin = 1e-3 dt0 */
+    double density;           /* mass m_i = density * 4/3 pi r_i^3 (reduced)    */
+    double alpha0;            /* 0.1 in the reference loop (:349); 0.25 in SetFIREData */
+    double dtgrow, dtshrink, alpha_shrink, dtmax_factor, dtmin_factor; /* 1.1, 0.5, 0.99, 100, 1e-3 */
+    int32_t ndelay;           /* 20                                              */
+    int32_t container;        /* rf_container                                    */
+    int64_t np_negative_max;  /* 30000                                           */
+    int64_t max_steps;        /* run at least this many steps (MaxStepsRelaxation) */
+    int64_t hard_cap;         /* never more than this many steps (0: 10 max_steps) */
+    double center[3];
+    double radius, half_height, hide_z;   /* container; hide_z: z above which a
+                                             particle counts as removed       */
+    double hidden[3];         /* where removed particles are put              */
+    double gap_tol;           /* stop only when min (rho - a)/a >= gap_tol (e.g. -0.01) */
+    int32_t check_every;      /* steps between gap checks                      */
+    int32_t delete_after;     /* from this step on, a check that fails removes the
+                                 partner of the worst pair (<0: never)         */
+    double skin;              /* neighbour-list skin (GPU only; never changes results) */
+    uint32_t quirks;          /* RF_QUIRK_FIRE_NO_INERTIA                       */
+    int32_t device;
+} rf_fire_cfg;
+
+typedef struct rf_fire_result {
+    int64_t steps;            /* FIRE steps executed                           */
+    int64_t np_positive, np_negative;
+    int64_t removed;          /* particles moved to the hidden point           */
+    double dt, alpha;         /* final FIRE state                              */
+    double min_gap;           /* last checked min (rho - a)/a                  */
+    double power;             /* last F.V                                      */
+    int32_t stop_reason;      /* 1: converged after max_steps, 2: too many negative
+                                 steps, 3: hard cap                            */
+    int32_t pad_;
+} rf_fire_result;
+
+/* Fill cfg with the reference's FIRE constants (SetFIREData, loop :349). */
+rf_status rf_fire_defaults(rf_fire_cfg* cfg);
+
+/* Relax n particles in place: x, y, z (mm) are read and overwritten, r are
+ * the radii.  `velocity` (3n: x block, y block, z block; may be null) is the
+ * initial velocity and receives the final one. */
+rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
+                        const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,8 @@ int64_t ora_sizeof(int32_t which) {
         case 6: return (int64_t)sizeof(rf_test_state);
         case 7: return (int64_t)sizeof(rf_counters);
         case 8: return (int64_t)sizeof(rf_state_view);
+        case 9: return (int64_t)sizeof(rf_fire_cfg);
+        case 10: return (int64_t)sizeof(rf_fire_result);
         default: return -1;
     }
 }

@@ -60,6 +60,13 @@ int ora_set_state(ora_ctx* c, const rf_state_view* v);
 int ora_get_partials(const ora_ctx* c, double* bottom, double* top, int64_t cap);
 
 /* Lognormal per-bond strength draw (also restated by the product host code). */
+/* FIRE relaxation of a packing (fire_oracle.c); same arguments as
+ * rf_fire_relax plus the OpenMP thread count. */
+int ora_fire_defaults(rf_fire_cfg* cfg);
+int ora_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
+                   const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out,
+                   int32_t nthreads);
+
 double ora_bond_strength(uint64_t seed, int64_t lo, int64_t hi, int32_t stream,
                          double mean, double cv);
 

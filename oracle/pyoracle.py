@@ -76,6 +76,10 @@ def lib():
     L.ora_bond_strength.restype = C.c_double
     L.ora_sizeof.argtypes = [C.c_int32]
     L.ora_sizeof.restype = C.c_int64
+    L.ora_fire_defaults.argtypes = [P(abi.FireCfg)]
+    L.ora_fire_relax.argtypes = [C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
+                                 P(C.c_double), P(abi.FireCfg), P(C.c_double),
+                                 P(abi.FireResult), C.c_int32]
     _lib = L
     return L
 
@@ -101,6 +105,35 @@ def box_reference(space, cell):
     if rc:
         raise RuntimeError(f"ora_box_reference failed {rc}")
     return b
+
+
+def fire_defaults():
+    abi = _abi()
+    c = abi.FireCfg()
+    lib().ora_fire_defaults(C.byref(c))
+    return c
+
+
+def fire_relax(xyz, r, cfg, vel=None, threads=1):
+    """FIRE relaxation on the CPU; returns (xyz, vel (n, 3), FireResult)."""
+    import numpy as np
+    abi = _abi()
+    xyz = np.asarray(xyz, dtype=np.float64)
+    n = len(xyz)
+    x = np.ascontiguousarray(xyz[:, 0]).copy()
+    y = np.ascontiguousarray(xyz[:, 1]).copy()
+    z = np.ascontiguousarray(xyz[:, 2]).copy()
+    rr = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.zeros(3 * n) if vel is None else np.ascontiguousarray(np.asarray(vel).T.reshape(-1))
+    v = v.astype(np.float64).copy()
+    out = abi.FireResult()
+    D = C.POINTER(C.c_double)
+    rc = lib().ora_fire_relax(n, x.ctypes.data_as(D), y.ctypes.data_as(D), z.ctypes.data_as(D),
+                              rr.ctypes.data_as(D), C.byref(cfg), v.ctypes.data_as(D),
+                              C.byref(out), int(threads))
+    if rc:
+        raise RuntimeError(f"ora_fire_relax failed {rc}")
+    return np.stack([x, y, z], axis=1), v.reshape(3, n).T.copy(), out
 
 
 class Oracle:

@@ -34,7 +34,8 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.parametrize("which,cls", [(0, abi.Material), (1, abi.Box), (2, abi.Sample),
                                        (3, abi.Options), (4, abi.Derived), (5, abi.PlatenCfg),
                                        (6, abi.TestState), (7, abi.Counters),
-                                       (8, abi.StateView)])
+                                       (8, abi.StateView), (9, abi.FireCfg),
+                                       (10, abi.FireResult)])
 def test_struct_layout(which, cls):
     assert pyoracle.lib().ora_sizeof(which) == C.sizeof(cls)
 
@@ -73,3 +74,7 @@ def test_create_without_device_fails_loudly():
                              gen_kwargs=dict(max_iter=10))
     with pytest.raises(_lib.RockfitError, match="RF_ERR_NODEVICE"):
         core.Context(w["xyz"], w["r"], w["mat"], w["box"])
+    from rockfit import fire
+    c = fire.defaults(D=1.0, dt0=1e-3, density=2.65, max_steps=1)
+    with pytest.raises(_lib.RockfitError, match="RF_ERR_NODEVICE"):
+        fire.relax(w["xyz"], w["r"], c)
