@@ -105,6 +105,17 @@ struct Ctrl {
     int64_t step, steps_stop, stepsave, last_bonds;
     int32_t loading, fracture, done, halt;
     int32_t cur, rebuild, err, pad;
+    int64_t stop_at;  // halt once step reaches this (0: no limit; batched runs)
+};
+
+// One sample of a batched launch: its state, constants, control block and
+// the first logical block of the step kernel that belongs to it.
+struct BatchEntry {
+    DevState s;
+    DevParams p;
+    Ctrl* ctrl;
+    int32_t blk0;
+    int32_t pad;
 };
 
 // Sets the text rf_last_error() returns and passes `st` through
@@ -119,10 +130,14 @@ hipError_t launch_construct_bonds(const DevState& s, const DevParams& p, Ctrl* c
 hipError_t launch_prune_initial(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_build_candidates(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_clear_flags(Ctrl* ctrl, hipStream_t st);
+hipError_t launch_set_stop(Ctrl* ctrl, int64_t stop_at, hipStream_t st);
 hipError_t launch_set_strengths(BondRec* brec, const double* sc, const double* tc, int64_t n,
                                 hipStream_t st);
 hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample, int32_t nblk_total,
+                             hipStream_t st);
+hipError_t launch_controller_batch(const BatchEntry* e, int32_t count, hipStream_t st);
 hipError_t launch_counters(const DevState& s, const DevParams& p, Ctrl* ctrl, int64_t* out3,
                            hipStream_t st);
 

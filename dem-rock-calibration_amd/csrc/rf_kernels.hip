@@ -353,15 +353,14 @@ __device__ __forceinline__ void contact_eval(const DevParams& p, const double4& 
 // its result masked afterwards, so a wave waits on two memory round trips
 // before any force work.  Contacts gather in rounds of one slot per lane.
 // Bonds beyond the 8th and appends are rare paths.
-__global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevParams p,
-                                                            Ctrl* __restrict__ ctrl) {
+__device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
+                                          Ctrl* __restrict__ ctrl, const int blk) {
     __shared__ double sh_b[kPerBlock];
     __shared__ double sh_t[kPerBlock];
     __shared__ int sh_nb[kPerBlock];
     if (ctrl->halt) return;  // uniform for the whole grid
     const int l = threadIdx.x & (kLanes - 1);
     const int grp = threadIdx.x / kLanes;
-    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t i = (int64_t)blk * kPerBlock + grp;
     const bool valid = i < p.n;
     const int cur = ctrl->cur;
@@ -801,13 +800,28 @@ __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevPar
     }
 }
 
+__global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevParams p,
+                                                            Ctrl* __restrict__ ctrl) {
+    step_body(s, p, ctrl, xcd_block((int)blockIdx.x, (int)gridDim.x));
+}
+
+// Many samples in one launch: logical block L belongs to sample
+// blk_sample[L] and is that sample's block L - blk0, so every sample runs
+// exactly as it would alone.
+__global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step_batch(
+    const BatchEntry* __restrict__ e, const int32_t* __restrict__ blk_sample) {
+    const int L = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const BatchEntry& b = e[blk_sample[L]];
+    step_body(b.s, b.p, b.ctrl, L - b.blk0);
+}
+
 // -------------------------------------------------------- controller ------
 // SumForcesULoading / SumForcesBLoading (md_results.cu:56-145) and
 // CompressU_S3 / CompressB_S3 (md_data_types.h:347-379, 462-494), on device.
 // Platen-force sum: kCtrlThreads strided running sums of the block partials,
 // then a kCtrlThreads-wide tree (the CPU restatement uses the same order).
-__global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevParams p,
-                                                             Ctrl* __restrict__ ctrl) {
+__device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
+                                          Ctrl* __restrict__ ctrl) {
     __shared__ double sb[kCtrlThreads];
     __shared__ double stt[kCtrlThreads];
     __shared__ long long sn[kCtrlThreads];
@@ -891,8 +905,19 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevPara
                         (fabs(c.strain) < p.max_strain && !c.fracture);
         if (!go) c.done = 1;
     }
-    c.halt = (c.done || c.rebuild || c.err) ? 1 : 0;
+    c.halt = (c.done || c.rebuild || c.err || (c.stop_at > 0 && c.step >= c.stop_at)) ? 1 : 0;
     *ctrl = c;
+}
+
+__global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevParams p,
+                                                             Ctrl* __restrict__ ctrl) {
+    ctrl_body(s, p, ctrl);
+}
+
+// One block per sample.
+__global__ __launch_bounds__(kCtrlThreads) void k_controller_batch(const BatchEntry* __restrict__ e) {
+    const BatchEntry& b = e[blockIdx.x];
+    ctrl_body(b.s, b.p, b.ctrl);
 }
 
 // ------------------------------------------------------ cell binning ------
@@ -1116,10 +1141,19 @@ __global__ void k_set_strengths(BondRec* brec, const double* sc, const double* t
     brec[q].tc = tc[q];
 }
 
+__global__ void k_set_stop(Ctrl* ctrl, int64_t stop_at) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctrl->stop_at = stop_at;
+        ctrl->halt = (ctrl->done || ctrl->err || ctrl->rebuild ||
+                      (stop_at > 0 && ctrl->step >= stop_at)) ? 1 : 0;
+    }
+}
+
 __global__ void k_clear_flags(Ctrl* ctrl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctrl->rebuild = 0;
-        ctrl->halt = (ctrl->done || ctrl->err) ? 1 : 0;
+        ctrl->halt = (ctrl->done || ctrl->err ||
+                      (ctrl->stop_at > 0 && ctrl->step >= ctrl->stop_at)) ? 1 : 0;
     }
 }
 
@@ -1201,6 +1235,11 @@ hipError_t launch_set_strengths(BondRec* brec, const double* sc, const double* t
     return hipGetLastError();
 }
 
+hipError_t launch_set_stop(Ctrl* ctrl, int64_t stop_at, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(64), 0, st, ctrl, stop_at);
+    return hipGetLastError();
+}
+
 hipError_t launch_clear_flags(Ctrl* ctrl, hipStream_t st) {
     hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, st, ctrl);
     return hipGetLastError();
@@ -1213,6 +1252,18 @@ hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStr
 
 hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st) {
     hipLaunchKernelGGL(k_controller, dim3(1), dim3(kCtrlThreads), 0, st, s, p, ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample, int32_t nblk_total,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(k_step_batch, dim3((unsigned)nblk_total), dim3(kBlock), 0, st, e,
+                       blk_sample);
+    return hipGetLastError();
+}
+
+hipError_t launch_controller_batch(const BatchEntry* e, int32_t count, hipStream_t st) {
+    hipLaunchKernelGGL(k_controller_batch, dim3((unsigned)count), dim3(kCtrlThreads), 0, st, e);
     return hipGetLastError();
 }
 

@@ -938,3 +938,206 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- batches ----
+// Several samples advance with one step launch and one controller launch
+// per step (k_step_batch, k_controller_batch).  Each sample keeps its own
+// arrays, constants and control block; a block-to-sample table routes the
+// blocks of the shared launch.  A sample stops at its own step target
+// (Ctrl::stop_at), at the end of its test, or for a list rebuild, which the
+// host then runs for that sample alone.
+
+struct rf_batch {
+    std::vector<rf_ctx*> c;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rf::BatchEntry* d_e = nullptr;
+    int32_t* d_map = nullptr;
+    int32_t nblk_total = 0;
+    int32_t chunk = 256;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+rf_status batch_upload(rf_batch* b) {
+    std::vector<rf::BatchEntry> e(b->c.size());
+    int32_t off = 0;
+    for (size_t k = 0; k < b->c.size(); ++k) {
+        const rf_ctx* c = b->c[k];
+        e[k].s = c->d;
+        e[k].p = c->p;
+        e[k].ctrl = c->ctrl;
+        e[k].blk0 = off;
+        e[k].pad = 0;
+        off += c->p.nblk;
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_e, e.data(), e.size() * sizeof(rf::BatchEntry),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return RF_OK;
+}
+
+rf_status batch_sync(rf_batch* b) {
+    for (rf_ctx* c : b->c)
+        HIP_TRY(hipMemcpyAsync(&c->hctrl, c->ctrl, sizeof(rf::Ctrl), hipMemcpyDeviceToHost,
+                               b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_batch_create(rf_ctx* const* ctxs, int32_t count, rf_batch** out) {
+    if (out) *out = nullptr;
+    if (!ctxs || count <= 0 || !out) return fail(RF_ERR_ARG, "rf_batch_create: bad arguments");
+    rf_batch* b = new rf_batch();
+    b->c.assign(ctxs, ctxs + count);
+    b->device = b->c[0]->device;
+    int64_t total = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        rf_ctx* c = b->c[k];
+        if (!c || !c->built || c->device != b->device) {
+            delete b;
+            return fail(RF_ERR_ARG, "batch member %d is null, not built or on another device", k);
+        }
+        for (int32_t q = 0; q < k; ++q)
+            if (b->c[q] == c) {
+                delete b;
+                return fail(RF_ERR_ARG, "context %d appears twice in the batch", k);
+            }
+        total += c->p.nblk;
+    }
+    if (total >= ((int64_t)1 << 31)) {
+        delete b;
+        return fail(RF_ERR_ARG, "batch too large");
+    }
+    b->nblk_total = (int32_t)total;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipMalloc(&b->d_e, count * sizeof(rf::BatchEntry));
+    if (e == hipSuccess) e = hipMalloc(&b->d_map, std::max<int64_t>(total, 1) * sizeof(int32_t));
+    if (e != hipSuccess) {
+        rf_batch_destroy(b);
+        return fail(RF_ERR_HIP, "batch allocation failed: %s", hipGetErrorString(e));
+    }
+    std::vector<int32_t> map((size_t)total);
+    int64_t off = 0;
+    for (int32_t k = 0; k < count; ++k)
+        for (int32_t q = 0; q < b->c[k]->p.nblk; ++q) map[off++] = k;
+    e = hipMemcpy(b->d_map, map.data(), total * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rf_batch_destroy(b);
+        return fail(RF_ERR_HIP, "batch upload failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return RF_OK;
+}
+
+rf_status rf_batch_destroy(rf_batch* b) {
+    if (!b) return RF_OK;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->d_e) hipFree(b->d_e);
+    if (b->d_map) hipFree(b->d_map);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return RF_OK;
+}
+
+rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
+    if (!b || nsteps < 0) return fail(RF_ERR_ARG, "rf_batch_step: bad arguments");
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t count = b->c.size();
+    for (rf_ctx* c : b->c) HIP_TRY(hipStreamSynchronize(c->stream));
+    rf_status st = batch_sync(b);
+    if (st) return st;
+    std::vector<int64_t> start(count);
+    for (size_t k = 0; k < count; ++k) {
+        start[k] = b->c[k]->hctrl.step;
+        const int64_t stop = nsteps > INT64_MAX / 4 ? INT64_MAX / 4 : start[k] + nsteps;
+        HIP_TRY(rf::launch_set_stop(b->c[k]->ctrl, stop, b->stream));
+    }
+    st = batch_upload(b);
+    if (st) return st;
+    st = batch_sync(b);
+    if (st) return st;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, b->stream));
+    rf_status result = RF_OK;
+    for (;;) {
+        bool active = false;
+        for (rf_ctx* c : b->c) active = active || !c->hctrl.halt;
+        if (!active) break;
+        std::vector<int64_t> before(count);
+        for (size_t k = 0; k < count; ++k) before[k] = b->c[k]->hctrl.step;
+        for (int32_t s = 0; s < b->chunk; ++s) {
+            HIP_TRY(rf::launch_step_batch(b->d_e, b->d_map, b->nblk_total, b->stream));
+            HIP_TRY(rf::launch_controller_batch(b->d_e, (int32_t)count, b->stream));
+        }
+        st = batch_sync(b);
+        if (st) { result = st; break; }
+        bool progress = false, rebuilt = false;
+        for (size_t k = 0; k < count && !result; ++k) {
+            rf_ctx* c = b->c[k];
+            progress = progress || c->hctrl.step != before[k];
+            if (c->hctrl.err) { result = device_error(c); break; }
+            if (c->hctrl.rebuild) {
+                st = rebuild(c);
+                if (st) { result = st; break; }
+                if (c->hctrl.err) { result = device_error(c); break; }
+                rebuilt = true;
+            }
+        }
+        if (result) break;
+        if (rebuilt) {
+            st = batch_upload(b);  // a rebuild may have narrowed the skin
+            if (st) { result = st; break; }
+        } else if (!progress) {
+            result = fail(RF_ERR_STATE, "batch halted without a reason");
+            break;
+        }
+    }
+    HIP_TRY(hipEventRecord(t1, b->stream));
+    HIP_TRY(hipEventSynchronize(t1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    b->last_ms = ms;
+    for (size_t k = 0; k < count; ++k) HIP_TRY(rf::launch_set_stop(b->c[k]->ctrl, 0, b->stream));
+    st = batch_sync(b);
+    if (st && !result) result = st;
+    if (done_steps)
+        for (size_t k = 0; k < count; ++k) done_steps[k] = b->c[k]->hctrl.step - start[k];
+    return result;
+}
+
+rf_status rf_batch_run(rf_batch* b, rf_test_state* out) {
+    if (!b) return fail(RF_ERR_ARG, "null batch");
+    for (rf_ctx* c : b->c)
+        if (!c->has_test || c->cfg.kind == RF_TEST_NONE)
+            return fail(RF_ERR_STATE, "rf_batch_run needs a platen test on every sample");
+    rf_status st = rf_batch_step(b, INT64_MAX, nullptr);
+    if (st) return st;
+    for (size_t k = 0; k < b->c.size(); ++k) {
+        rf_ctx* c = b->c[k];
+        if (!c->hctrl.done) return fail(RF_ERR_STATE, "batch sample %zu did not finish", k);
+        st = post_process(c);
+        if (st) return st;
+        if (out) fill_state(c, out + k);
+    }
+    return RF_OK;
+}
+
+rf_status rf_batch_last_ms(rf_batch* b, double* total_ms) {
+    if (!b || !total_ms) return fail(RF_ERR_ARG, "null argument");
+    *total_ms = b->last_ms;
+    return RF_OK;
+}
+
+}  // extern "C"

@@ -27,6 +27,7 @@ EXPORTS = [
     "rf_get_bond_counts", "rf_get_counters", "rf_get_state", "rf_set_state",
     "rf_num_particles", "rf_num_slots", "rf_set_timing", "rf_last_timing", "rf_set_chunk",
     "rf_fire_defaults", "rf_fire_relax",
+    "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
 ]
 
 
@@ -81,6 +82,16 @@ def lib():
     L.rf_set_timing.restype = st
     L.rf_last_timing.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_double), P(C.c_int64)]
     L.rf_last_timing.restype = st
+    L.rf_batch_create.argtypes = [P(vp), C.c_int32, P(vp)]
+    L.rf_batch_create.restype = st
+    L.rf_batch_destroy.argtypes = [vp]
+    L.rf_batch_destroy.restype = st
+    L.rf_batch_step.argtypes = [vp, C.c_int64, P(C.c_int64)]
+    L.rf_batch_step.restype = st
+    L.rf_batch_run.argtypes = [vp, P(abi.TestState)]
+    L.rf_batch_run.restype = st
+    L.rf_batch_last_ms.argtypes = [vp, P(C.c_double)]
+    L.rf_batch_last_ms.restype = st
     L.rf_fire_defaults.argtypes = [P(abi.FireCfg)]
     L.rf_fire_defaults.restype = st
     L.rf_fire_relax.argtypes = [C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),

@@ -106,8 +106,11 @@ def place_in_box(xyz, r, mat, space_factor=(1.6, 1.6, 1.2)):
     mp_calculateonesample.cu:91) with cell edge 2*acut.  Returns (xyz, box)."""
     lo, hi = xyz.min(axis=0), xyz.max(axis=0)
     size = hi - lo + 2.0 * float(r.max())
-    space = [float(size[k] * space_factor[k]) for k in range(3)]
-    box = box_reference(space, 2.0 * mat.acut)
+    cell = 2.0 * mat.acut
+    # at least two cells of margin on each side, so small specimens keep their
+    # neighbour stencil inside the grid
+    space = [float(max(size[k] * space_factor[k], size[k] + 4.0 * cell)) for k in range(3)]
+    box = box_reference(space, cell)
     shift = np.array(box.center[:]) - 0.5 * (lo + hi)
     return xyz + shift, box
 

@@ -153,3 +153,36 @@ class Context:
         check(lib().rf_last_timing(self.h, C.byref(k), C.byref(n), C.byref(t), C.byref(s)),
               "rf_last_timing")
         return k.value, n.value, t.value, s.value
+
+
+class Batch:
+    """Several contexts advanced together (rf_batch_*): one step launch and
+    one controller launch per step for all samples; each sample's results
+    are the same as when it runs alone."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        check(lib().rf_batch_create(arr, len(self.contexts), C.byref(h)), "rf_batch_create")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rf_batch_destroy(self.h)
+            self.h = None
+
+    def step(self, nsteps):
+        done = (C.c_int64 * len(self.contexts))()
+        check(lib().rf_batch_step(self.h, int(nsteps), done), "rf_batch_step")
+        return list(done)
+
+    def run(self):
+        out = (abi.TestState * len(self.contexts))()
+        check(lib().rf_batch_run(self.h, out), "rf_batch_run")
+        return list(out)
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().rf_batch_last_ms(self.h, C.byref(ms)), "rf_batch_last_ms")
+        return ms.value

@@ -238,6 +238,27 @@ rf_status rf_last_timing(rf_ctx* ctx, double* step_kernel_ms, int64_t* timed_lau
 rf_status rf_set_chunk(rf_ctx* ctx, int32_t steps_per_sync);
 
 /* ------------------------------------------------------------------------
+ * Batches: several samples advanced together, one step launch and one
+ * controller launch per step for all of them.  No GPU counterpart in the
+ * reference: it replaces running samples one after another
+ * (TestSeveralSamples, mp_testseveralsamples.cu:119-161) or as separate
+ * processes of a farm (NN_MP.py:71-215).  Every sample evolves exactly as it
+ * would alone (same results bit for bit).  The contexts must be built, on
+ * one device, and are not to be used on their own while the batch is used.
+ * ------------------------------------------------------------------------ */
+typedef struct rf_batch rf_batch;
+
+rf_status rf_batch_create(rf_ctx* const* ctxs, int32_t count, rf_batch** out);
+rf_status rf_batch_destroy(rf_batch* batch);
+/* Advance every sample by up to nsteps steps (a sample stops early when its
+ * test ends); done_steps (count entries, may be null) receives each count. */
+rf_status rf_batch_step(rf_batch* batch, int64_t nsteps, int64_t* done_steps);
+/* Run every sample's platen test to its end; out has count entries. */
+rf_status rf_batch_run(rf_batch* batch, rf_test_state* out);
+/* Wall time (ms) of the last rf_batch_step / rf_batch_run on the device. */
+rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
+
+/* ------------------------------------------------------------------------
  * FIRE relaxation of a packing (sample preparation).
  *
  * Replaces the relaxation loop of h_CreateCylinderSample
