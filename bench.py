@@ -23,8 +23,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dem-rock-calibration_amd"))
 
 HBM_PEAK_GBPS = 8000.0
-# Every 8th step-kernel launch of the timed region is bracketed by HIP events
-# (bracketing all of them slows the whole loop by the events' own latency).
+# Step-kernel launches of the timed region are sampled with HIP events: each
+# 256-step chunk runs as a captured HIP graph of 255 steps plus one step
+# launched on its own between two events on the same stream.
 TIMING_STRIDE = 8
 # Measured HBM traffic of one step-kernel launch (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE, scaled by a copy kernel of known size), for the default workload.
@@ -89,6 +90,85 @@ def cpu_baseline(w, steps, threads):
                        f"after 2 warm-up steps, FP64 CPU oracle, OpenMP {threads} threads")
 
 
+def batch_bench(count, steps, warmup):
+    """Many samples at once (rf_batch_*) against the same samples run one after
+    another on the GPU (the reference's TestSeveralSamples pattern,
+    mp_testseveralsamples.cu:119-161).  Samples: N ~ log-uniform [2k, 40k],
+    r ~ U[0.20, 0.30] mm, Latin hypercube over bond E [5, 50] GPa, strengths
+    [10, 100] MPa and friction [0.3, 0.8]; packings relaxed by FIRE on the
+    GPU; uniaxial loading at 0.1/s.  Prints one JSON line."""
+    import numpy as np
+
+    from rockfit import config
+    from rockfit.core import Batch, Context
+
+    rng = np.random.default_rng(20261203 + 4)
+    ns = np.exp(rng.uniform(np.log(2000.0), np.log(40000.0), count)).astype(int)
+
+    def lhs(lo, hi):
+        u = (rng.permutation(count) + rng.uniform(0.0, 1.0, count)) / count
+        return lo + (hi - lo) * u
+
+    E, sig, mu = lhs(5.0, 50.0), lhs(10.0, 100.0), lhs(0.3, 0.8)
+    t0 = time.perf_counter()
+    ws = []
+    for k in range(count):
+        w = config.make_workload(n=int(ns[k]), r_min=0.20, r_max=0.30, test="uniaxial",
+                                 seed=20261203 + 100 + k, gen_kwargs=dict(relax="gpu"),
+                                 mat_kwargs=dict(E_bond_gpa=float(E[k]), sigma_mpa=float(sig[k]),
+                                                 tau_mpa=float(1.5 * sig[k]),
+                                                 friction=float(mu[k])))
+        h = w["cfg"].size[2]
+        w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=-0.1 * h * 1e-3, dnsave=100)
+        ws.append(w)
+    total_n = int(sum(w["n"] for w in ws))
+    log(f"{count} samples, {total_n} particles, generated in {time.perf_counter() - t0:.1f}s")
+
+    def make():
+        out = []
+        for w in ws:
+            c = Context(w["xyz"], w["r"], w["mat"], w["box"])
+            c.build_bonds()
+            c.set_test(w["cfg"])
+            out.append(c)
+        return out
+
+    seq = make()
+    for c in seq:
+        c.step(warmup)
+    device_sync()
+    t1 = time.perf_counter()
+    seq_steps = sum(c.step(steps) for c in seq)
+    device_sync()
+    t_seq = time.perf_counter() - t1
+    del seq
+    bat = make()
+    b = Batch(bat)
+    b.step(warmup)
+    device_sync()
+    t1 = time.perf_counter()
+    done = b.step(steps)
+    device_sync()
+    t_bat = time.perf_counter() - t1
+    bat_steps = sum(done)
+    pst_bat = sum(w["n"] * d for w, d in zip(ws, done)) / t_bat
+    pst_seq = total_n * (seq_steps / count) / t_seq
+    log(f"batched {t_bat:.3f}s ({bat_steps} sample-steps), sequential {t_seq:.3f}s "
+        f"({seq_steps} sample-steps)")
+    print(json.dumps({
+        "metric": f"particle-steps/s, {count}-sample batch (N log-U[2k,40k]) on 1 GPU",
+        "value": pst_bat, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic (seeded generator, FIRE-relaxed on the GPU)",
+        "config": {"workload": "batch of uniaxial tests", "samples": count,
+                   "particles": total_n},
+        "sequential": {"value": pst_seq, "unit": "particle-steps/s",
+                       "kind": "same samples one after another on the GPU"},
+        "speedup": pst_bat / pst_seq,
+        "sample_steps_per_s": {"batched": bat_steps / t_bat, "sequential": seq_steps / t_seq},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,10 +179,15 @@ def main():
                     help="CPU-baseline steps (default: sized to ~15 s of CPU work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="measure a batch of this many samples against sequential runs and exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.batch > 0:
+        batch_bench(args.batch, args.steps, args.warmup)
+        return
     if args.copy_probe > 0:
         import torch
 

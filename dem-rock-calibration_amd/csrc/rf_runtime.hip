@@ -98,6 +98,12 @@ struct rf_ctx {
     std::vector<void*> allocs;
     int32_t chunk = 256;
     int32_t timing = 0;  // bracket every timing-th step launch with events (0: off)
+    // A captured chunk of `graph_steps` (step, controller) launch pairs,
+    // replayed while the constants it captured are unchanged.
+    bool use_graph = true;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_steps = 0;
+    rf::DevParams graph_p{};
     std::vector<hipEvent_t> ev;
     double last_kernel_ms = 0.0, last_total_ms = 0.0;
     int64_t last_steps = 0, last_timed = 0;
@@ -349,6 +355,7 @@ rf_status rf_destroy(rf_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     for (void* ptr : c->allocs) hipFree(ptr);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -377,6 +384,37 @@ rf_status device_error(rf_ctx* c) {
         return fail(RF_ERR_OVERFLOW, "candidate list overflow (more than %d within acut+skin)",
                     c->C);
     return fail(RF_ERR_STATE, "device error flags 0x%x", e);
+}
+
+// Launch `k` (step, controller) pairs as one captured graph.  The kernels
+// read the step's buffers and halt flag from the device control block, so the
+// same graph serves every chunk; it is captured again only when the
+// constants passed by value (DevParams) or the chunk length change.
+rf_status graph_chunk(rf_ctx* c, int64_t k) {
+    if (!c->graph_exec || c->graph_steps != k ||
+        std::memcmp(&c->graph_p, &c->p, sizeof(rf::DevParams)) != 0) {
+        if (c->graph_exec) {
+            HIP_TRY(hipGraphExecDestroy(c->graph_exec));
+            c->graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int64_t s = 0; s < k; ++s) {
+            rf::launch_step(c->d, c->p, c->ctrl, c->stream);
+            rf::launch_controller(c->d, c->p, c->ctrl, c->stream);
+        }
+        HIP_TRY(hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->graph_exec = nullptr;
+            return fail(RF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        c->graph_steps = k;
+        c->graph_p = c->p;
+    }
+    HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    return RF_OK;
 }
 
 // Cell sort of the current positions and fresh candidate lists.  A list
@@ -542,6 +580,12 @@ rf_status rf_set_timing(rf_ctx* c, int32_t enable) {
     return RF_OK;
 }
 
+rf_status rf_set_graph(rf_ctx* c, int32_t enable) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    c->use_graph = enable != 0;
+    return RF_OK;
+}
+
 rf_status rf_set_chunk(rf_ctx* c, int32_t steps) {
     if (!c || steps <= 0) return fail(RF_ERR_ARG, "bad chunk");
     c->chunk = steps;
@@ -579,17 +623,38 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
             }
         }
         const int64_t before = c->hctrl.step;
-        for (int64_t s = 0; s < k; ++s) {
-            const bool tm = c->timing && (before + s) % c->timing == 0;
-            if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s], c->stream));
-            HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
-            if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s + 1], c->stream));
-            HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+        // A full chunk runs as a captured graph; with timing on, its last step
+        // is launched on its own between two events (one sample per chunk).
+        const bool graphed = c->use_graph && k == c->chunk && k > 1;
+        if (graphed) {
+            st = graph_chunk(c, c->timing ? k - 1 : k);
+            if (st) return st;
+            if (c->timing) {
+                HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+                HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
+                HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+                HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+            }
+        } else {
+            for (int64_t s = 0; s < k; ++s) {
+                const bool tm = c->timing && (before + s) % c->timing == 0;
+                if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s], c->stream));
+                HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
+                if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s + 1], c->stream));
+                HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+            }
         }
         st = sync_ctrl(c);
         if (st) return st;
         const int64_t executed = c->hctrl.step - before;
-        if (c->timing)
+        if (c->timing && graphed) {
+            if (executed == k) {  // the sampled last step ran
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                kernel_ms += ms;
+                ++timed;
+            }
+        } else if (c->timing) {
             for (int64_t s = 0; s < executed; ++s) {
                 if ((before + s) % c->timing) continue;
                 float ms = 0.f;
@@ -597,6 +662,7 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
                 kernel_ms += ms;
                 ++timed;
             }
+        }
         remaining -= executed;
         if (c->hctrl.err) return device_error(c);
         if (c->hctrl.rebuild) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "rf_build_bonds", "rf_set_test", "rf_step", "rf_run", "rf_get_test_state", "rf_get_curve",
     "rf_get_bond_counts", "rf_get_counters", "rf_get_state", "rf_set_state",
     "rf_num_particles", "rf_num_slots", "rf_set_timing", "rf_last_timing", "rf_set_chunk",
-    "rf_fire_defaults", "rf_fire_relax",
+    "rf_fire_defaults", "rf_fire_relax", "rf_set_graph",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
 ]
 
@@ -82,6 +82,8 @@ def lib():
     L.rf_set_timing.restype = st
     L.rf_last_timing.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_double), P(C.c_int64)]
     L.rf_last_timing.restype = st
+    L.rf_set_graph.argtypes = [vp, C.c_int32]
+    L.rf_set_graph.restype = st
     L.rf_batch_create.argtypes = [P(vp), C.c_int32, P(vp)]
     L.rf_batch_create.restype = st
     L.rf_batch_destroy.argtypes = [vp]

@@ -143,6 +143,10 @@ class Context:
         """Bracket every `every`-th step-kernel launch with HIP events (0: off)."""
         check(lib().rf_set_timing(self.h, int(every)), "rf_set_timing")
 
+    def set_graph(self, on=True):
+        """Replay full rf_step chunks as captured HIP graphs (default on)."""
+        check(lib().rf_set_graph(self.h, 1 if on else 0), "rf_set_graph")
+
     def set_chunk(self, steps):
         check(lib().rf_set_chunk(self.h, int(steps)), "rf_set_chunk")
 

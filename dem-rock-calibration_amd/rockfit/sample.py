@@ -60,12 +60,28 @@ def _container_push(p, r, R, H):
         p[over, 1] *= s
 
 
+def _relax_gpu(pts, r, r_min, r_max, R, H, tol, max_iter):
+    """Overlap removal by standard FIRE on the GPU: repulsion of overlapping
+    pairs, centres held in the cylinder of radius R - r_max and half height
+    H/2 - r_max; stops once no pair overlaps by more than tol r_min."""
+    from . import abi, fire
+
+    c = fire.defaults(D=1.0, dt0=0.02, density=2.65, quirks=0, max_steps=max_iter,
+                      hard_cap=100 * max(max_iter, 1), gap_tol=-0.5 * tol * r_min / r_max,
+                      check_every=10, container=abi.RF_CONTAINER_CYLINDER_Z,
+                      radius=R - r_max, half_height=0.5 * H - r_max, hide_z=1e30)
+    xyz, _, _ = fire.relax(pts, r, c)
+    return xyz
+
+
 def generate(n, r_min, r_max, shape="cylinder", aspect=2.0, packing=0.60, seed=20261203,
-             tol=0.02, max_iter=300, jitter=0.05, step=0.5, verbose=False):
+             tol=0.02, max_iter=300, jitter=0.05, step=0.5, verbose=False, relax="cpu"):
     """Return (xyz[n,3], r[n]) in mm, centred at the origin.
 
     shape "cylinder": height = aspect * diameter (axis z).
     shape "disc": thickness = aspect * diameter (axis z), e.g. aspect 0.5.
+    relax "cpu": Jacobi overlap relaxation here; "gpu": FIRE relaxation on the
+    GPU (rf_fire_relax) inside the cylinder that holds the sphere centres.
     """
     from scipy.spatial import cKDTree
 
@@ -91,6 +107,8 @@ def generate(n, r_min, r_max, shape="cylinder", aspect=2.0, packing=0.60, seed=2
     perm = rng.permutation(n)
     r = r[perm]
     _container_push(pts, r, R, H)
+    if relax == "gpu":
+        return _relax_gpu(pts, r, r_min, r_max, R, H, tol, max_iter), r
     skin = 0.3 * r_min
     pairs = None
     ref = None

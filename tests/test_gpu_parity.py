@@ -127,3 +127,30 @@ def test_brazilian_parity():
     compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="brazilian")
     compare_test_states(g.test_state(), o.test_state(), what="brazilian")
     np.testing.assert_array_equal(g.curve(), o.curve())
+
+
+def test_graph_replay_parity():
+    """Full chunks replayed as captured HIP graphs (with and without timing
+    events inside) give the same state as plain launches, over several
+    replays with host work in between."""
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="uniaxial", seed=5,
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=5, max_steps=2000,
+                                              max_strain=0.05))
+    ctxs = []
+    for graph, timing in ((True, 0), (False, 0), (True, 4)):  # timing: one sample per chunk
+        c = Context(w["xyz"], w["r"], w["mat"], w["box"], skin=0.03)
+        c.build_bonds()
+        c.set_test(w["cfg"])
+        c.set_graph(graph)
+        c.set_timing(timing)
+        ctxs.append(c)
+    for chunk in (600, 300, 400):
+        done = [c.step(chunk) for c in ctxs]
+        assert done[0] == done[1] == done[2]
+        ref = ctxs[1].get_state()
+        for k in (0, 2):
+            compare_states(ctxs[k].get_state(), ref, w["n"], 32, what=f"graph case {k}")
+            compare_test_states(ctxs[k].test_state(), ctxs[1].test_state())
+    kms, timed, _, _ = ctxs[2].last_timing()
+    assert timed > 0 and kms > 0.0
+    assert ctxs[0].counters().rebuilds > 1
