@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["csrc/rf_kernels.hip", "csrc/rf_runtime.hip", "csrc/rf_fire.hip"]
+SOURCES = ["csrc/rf_kernels.hip", "csrc/rf_runtime.hip", "csrc/rf_fire.hip", "csrc/rf_sens.hip"]
 OUT = os.path.join(HERE, "lib", "librockfit_dem.so")
 ARCH = os.environ.get("RF_OFFLOAD_ARCH", "gfx950")
 

@@ -27,6 +27,10 @@ constexpr int kErrRowOverflow = 1;   // no free slot for a new contact
 constexpr int kErrCandOverflow = 2;  // candidate list longer than cand_cap
 constexpr int kErrGrid = 4;          // particle outside the padded cell grid
 constexpr int kErrCurve = 8;         // curve buffer full
+constexpr int kErrSensList = 16;     // a sensitivity window would change the slot table
+
+// Parameter directions carried by one sensitivity run.
+constexpr int kSensK = 4;
 
 // One bond of a particle's bond table: 128 bytes, the 12 history values
 // first (written back each step), then the constants.
@@ -107,6 +111,26 @@ struct Ctrl {
     int32_t cur, rebuild, err, pad;
     int64_t stop_at;  // halt once step reaches this (0: no limit; batched runs)
 };
+
+// Tangent state of a sensitivity run: kSensK directions, direction k of
+// particle i at [k*n + i], of bond entry q at [(k*n*B + q)*12 + u] (the 12
+// history values), of slot q at [k*n*S + q].  dmat[k] holds the derivatives
+// of (E, G, hm_E, hm_G, mu, mu_r, p_A, E_wall, G_wall, bond radius ratio,
+// bond radius) along direction k.
+struct SensState {
+    double4* tpos[2];
+    double4* tvel[2];
+    double4* tang[2];
+    double* tbrec;
+    double4* tot;
+    double* tpart_b;
+    double* tpart_t;
+    double* tcurve;   // [point][k][bottom, top] stress tangents
+    double dmat[kSensK][11];
+};
+
+hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
+                            hipStream_t st);
 
 // One sample of a batched launch: its state, constants, control block and
 // the first logical block of the step kernel that belongs to it.

@@ -29,14 +29,12 @@
 // IEEE-rounded FP64 operations; expressions are evaluated left to right as
 // written, in the same order as the CPU restatement.
 #include "rf_internal.h"
+#include "rf_physics.h"
 
 #include "../../include/rockfit_dem.h"
 
 namespace rf {
 
-#define RF_PI 3.1415926535897932384626433832795
-constexpr double kDamp = -1.8257;          // -2 sqrt(5/6), md_calculateforcesdem3.cu:248
-constexpr double kTwoThirds = 2.0 / 3.0;   // MCf_2d3
 
 // Position of a stencil cell in the reference's visiting order, indexed by
 // (dx | dy << 1 | dz << 2) where d = 1 means "the neighbour on the particle's
@@ -90,63 +88,16 @@ __device__ __forceinline__ int free_slot(uint32_t freem, int r, bool q0) {
     return r < __popc(freem) ? nth_bit(freem, r) : -1;
 }
 
-struct WallOut {
-    double fnx, fny, fnz, ftx, fty, ftz, mtx, mty, mtz, max_, may, maz, ox, oy, oz;
-};
+// Double-precision entry points of the pair physics (rf_physics.h).
+using WallOut = WallOutT<double>;
 
-// Particle-platen Hertz-Mindlin, dd_Calculate_fijn_pwHM_2 +
-// dd_Calculate_fijtmijt_pwHM_2 (md_compression_hm.cu:20-154): R* = r_i,
-// m* = m_i, previous tangential overlap (opx, opy, opz) including the
-// reference's o.x*o.y rescale typo (:91).
 __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, double mi,
                                              const double4& vi, const double4& wi,
                                              double nwx, double nwy, double nwz,
                                              double wvx, double wvy, double wvz, double d,
                                              double opx, double opy, double opz, WallOut& o) {
-    double c1 = sqrt(ri * fabs(ri - d));
-    const double Kn = 2.0 * p.Ew * c1;
-    const double Kt = 8.0 * p.Gw * c1;
-    const double vx = wvx - vi.x + ri * (nwy * wi.z - nwz * wi.y);
-    const double vy = wvy - vi.y + ri * (nwz * wi.x - nwx * wi.z);
-    const double vz = wvz - vi.z + ri * (nwx * wi.y - nwy * wi.x);
-    const double vn = nwx * vx + nwy * vy + nwz * vz;
-    const double fnm = (d - ri) * Kn * kTwoThirds;
-    c1 = kDamp * p.pA * vn * sqrt(Kn * mi);
-    o.fnx = nwx * (fnm + c1); o.fny = nwy * (fnm + c1); o.fnz = nwz * (fnm + c1);
-    const double vtx = vx - nwx * vn, vty = vy - nwy * vn, vtz = vz - nwz * vn;
-    c1 = nwx * opx + nwy * opy + nwz * opz;
-    double ox = opx - nwx * c1, oy = opy - nwy * c1, oz = opz - nwz * c1;
-    c1 = ox * ox + oy * oy + oz * oz;
-    if (c1 > 1e-18) {
-        c1 = 1.0 / c1;
-        c1 *= opx * opy + opy * opy + opz * opz;
-        c1 = sqrt(c1);
-        ox *= c1; oy *= c1; oz *= c1;
-    }
-    ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
-    double ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
-    const double ftm = ftx * ftx + fty * fty + ftz * ftz;
-    if (ftm > p.mu * p.mu * fnm * fnm) {
-        c1 = p.mu * fabs(fnm) * (1.0 / sqrt(ftm));
-        ftx *= c1; fty *= c1; ftz *= c1;
-        c1 = 1.0 / Kt;
-        ox = ftx * c1; oy = fty * c1; oz = ftz * c1;
-    } else {
-        c1 = kDamp * p.pA * sqrt(Kt * mi);
-        ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
-    }
-    o.ox = ox; o.oy = oy; o.oz = oz;
-    o.ftx = ftx; o.fty = fty; o.ftz = ftz;
-    o.mtx = (nwy * ftz - nwz * fty) * ri;
-    o.mty = (nwz * ftx - nwx * ftz) * ri;
-    o.mtz = (nwx * fty - nwy * ftx) * ri;
-    const double wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
-    if (wim > 1e-18) {
-        c1 = -p.mur * fabs(fnm) * (1.0 / sqrt(wim));
-        o.max_ = wi.x * c1; o.may = wi.y * c1; o.maz = wi.z * c1;
-    } else {
-        o.max_ = 0.0; o.may = 0.0; o.maz = 0.0;
-    }
+    wall_contact_t<double>(mat_of(p), p, ri, mi, v4(vi), v4(wi), nwx, nwy, nwz, wvx, wvy, wvz, d,
+                           opx, opy, opz, 0.0, o);
 }
 
 // ------------------------------------------------------------ step kernel --
@@ -181,165 +132,30 @@ __device__ __forceinline__ int xcd_block(int w, int nblk) {
 #endif
 }
 
-// One parallel bond seen from particle i (d_CalculateForcesDEM_22 and the
-// dd_*_2 helpers, md_calculateforcesdem3.cu:20-200, 427-486): rotates the
-// stored history into the current frame, increments it, and returns the
-// force and moment contributions, the new history h[0..5] (double2 pairs
-// in BondRec order) and whether the bond breaks.
 __device__ __forceinline__ bool bond_eval(const DevParams& p, const double4& pi, const double4& vi,
                                           const double4& wi, const double4& pj, const double4& vj,
                                           const double4& wj, const double2 (&r)[8],
                                           double (&c)[6], double2 (&h)[6]) {
-    const double ri = pi.w, rj = pj.w;
-    const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-    const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-    const double inv = 1.0 / rijm;
-    const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
-    const double pnx = r[0].x, pny = r[0].y, pnz = r[1].x;
-    const double oo0 = r[1].y, oo1 = r[2].x, oo2 = r[2].y;
-    const double on0 = r[3].x, on1 = r[3].y, on2 = r[4].x;
-    const double ot0 = r[4].y, ot1 = r[5].x, ot2 = r[5].y;
-    const double lam = r[6].x, bsc = r[6].y, btc = r[7].x, L0 = r[7].y;
-    const double bb = p.b_ratio > 0.0 ? p.b_ratio * (ri < rj ? ri : rj) : p.b_abs;
-    double cefn = p.E * (rijm - L0) * lam;
-    double Sc = cefn;
-    cefn *= bb * bb * RF_PI;
-    const double tx = pny * nz - pnz * ny;
-    const double ty = pnz * nx - pnx * nz;
-    const double tz = pnx * ny - pny * nx;
-    const double cw = ((wj.x + wi.x) * nx + (wj.y + wi.y) * ny + (wj.z + wi.z) * nz) * p.dt * 0.5;
-    const double px = nx * cw, py = ny * cw, pz = nz * cw;
-    const double M1x = 1.0 + tz * pz + ty * py, M1y = pz - tz - ty * px, M1z = -py - tz * px + ty;
-    const double M2x = tz - pz - tx * py, M2y = tz * pz + 1.0 + tx * px, M2z = -tz * py + px - tx;
-    const double M3x = -ty - tx * pz + py, M3y = -ty * pz + tx - px, M3z = ty * py + tx * px + 1.0;
-    // torsion
-    const double dwx = wj.x - wi.x, dwy = wj.y - wi.y, dwz = wj.z - wi.z;
-    double c1 = nx * dwx + ny * dwy + nz * dwz;
-    c1 *= p.dt * 0.5 * p.G * lam;
-    const double mnx = M1x * on0 + M1y * on1 + M1z * on2 + nx * c1;
-    const double mny = M2x * on0 + M2y * on1 + M2z * on2 + ny * c1;
-    const double mnz = M3x * on0 + M3y * on1 + M3z * on2 + nz * c1;
-    const double J4 = RF_PI * bb * bb * bb * bb;
-    // bending
-    const double c2 = p.dt * 0.25 * p.E * lam;
-    const double c1b = nx * dwx + ny * dwy + nz * dwz;
-    const double mtx = M1x * ot0 + M1y * ot1 + M1z * ot2 + (dwx - nx * c1b) * c2;
-    const double mty = M2x * ot0 + M2y * ot1 + M2z * ot2 + (dwy - ny * c1b) * c2;
-    const double mtz = M3x * ot0 + M3y * ot1 + M3z * ot2 + (dwz - nz * c1b) * c2;
-    const double c2J = (p.quirks & RF_QUIRK_BEND_DOUBLE_SCALE) ? c2 * J4 : J4;
-    // shear force and asymmetric moment
-    const double hx = 0.5 * (wj.x + wi.x), hy = 0.5 * (wj.y + wi.y), hz = 0.5 * (wj.z + wi.z);
-    const double vx = vj.x - vi.x - hy * rz + hz * ry;
-    const double vy = vj.y - vi.y - hz * rx + hx * rz;
-    const double vz = vj.z - vi.z - hx * ry + hy * rx;
-    const double cn = nx * vx + ny * vy + nz * vz;
-    const double ox = (M1x * oo0 + M1y * oo1 + M1z * oo2) + (vx - nx * cn) * p.dt;
-    const double oy = (M2x * oo0 + M2y * oo1 + M2z * oo2) + (vy - ny * cn) * p.dt;
-    const double oz = (M3x * oo0 + M3y * oo1 + M3z * oo2) + (vz - nz * cn) * p.dt;
-    double cef = p.G * lam;
-    double Mc = cef;
-    cef *= bb * bb * RF_PI;
-    c[0] = nx * cefn + ox * cef;
-    c[1] = ny * cefn + oy * cef;
-    c[2] = nz * cefn + oz * cef;
-    c[3] = (mnx * J4 + mtx * c2J) + 0.5 * (ry * oz - rz * oy) * cef;
-    c[4] = (mny * J4 + mty * c2J) + 0.5 * (rz * ox - rx * oz) * cef;
-    c[5] = (mnz * J4 + mtz * c2J) + 0.5 * (rx * oy - ry * ox) * cef;
-    // Breakage: Sc = Sc0 + |m_t| 4b > sigma_c or Mc = (G/L0)|o_t| + |m_n| 2b > tau_c.
-    // The three norms are first taken in single precision (relative error
-    // below 2e-7); when both sums stay below their strengths by more than
-    // 1e-6 of the norm terms the exact FP64 test cannot succeed either, and
-    // is skipped.  Otherwise the exact test decides.
-    const double os2 = ox * ox + oy * oy + oz * oz;
-    const double mt2 = mtx * mtx + mty * mty + mtz * mtz;
-    const double mn2 = mnx * mnx + mny * mny + mnz * mnz;
-    const double tS = (double)__builtin_amdgcn_sqrtf((float)mt2) * 4.0 * bb;
-    const double tM1 = Mc * (double)__builtin_amdgcn_sqrtf((float)os2);
-    const double tM2 = (double)__builtin_amdgcn_sqrtf((float)mn2) * 2.0 * bb;
-    const double Se = Sc + tS, Me = tM1 + tM2;
-    const bool certain = (Se + 1e-6 * tS + 1e-14 * (fabs(Se) + bb) < bsc) &
-                         (Me + 1e-6 * Me + 1e-14 * (Me + bb) < btc);
-    bool brk = false;
-    if (!certain) {
-        Mc *= sqrt(os2);
-        Sc += sqrt(mt2) * 4.0 * bb;
-        Mc += sqrt(mn2) * 2.0 * bb;
-        brk = Sc > bsc || Mc > btc;
-    }
-    h[0] = make_double2(nx, ny);
-    h[1] = make_double2(nz, ox);
-    h[2] = make_double2(oy, oz);
-    h[3] = make_double2(mnx, mny);
-    h[4] = make_double2(mnz, mtx);
-    h[5] = make_double2(mty, mtz);
+    const double hist[12] = {r[0].x, r[0].y, r[1].x, r[1].y, r[2].x, r[2].y,
+                             r[3].x, r[3].y, r[4].x, r[4].y, r[5].x, r[5].y};
+    double hn[12];
+    const bool brk = bond_eval_t<double>(mat_of(p), p, v4(pi), v4(vi), v4(wi), v4(pj), v4(vj),
+                                         v4(wj), hist, r[6].x, r[7].y, r[6].y, r[7].x, c, hn);
+#pragma unroll
+    for (int u = 0; u < 6; ++u) h[u] = make_double2(hn[2 * u], hn[2 * u + 1]);
     return brk;
 }
 
-// One Hertz-Mindlin contact seen from particle i (d_CalculateForcesDEM_21 and
-// dd_*ppHM_2, md_calculateforcesdem3.cu:202-425) for a pair known to overlap
-// (rijm < ri + rj).  Returns the force and moment contributions and the new
-// tangential overlap.
 __device__ __forceinline__ void contact_eval(const DevParams& p, const double4& pi,
                                              const double4& vi, const double4& wi,
                                              const double4& pj, const double4& vj,
                                              const double4& wj, const double4& oprev, double wim,
                                              bool roll, double (&c)[6], double4& onew) {
-    const double ri = pi.w, mi = vi.w, rj = pj.w, mj = vj.w;
-    const double rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
-    const double rijm = sqrt(rx * rx + ry * ry + rz * rz);
-    const double rsum = ri + rj;
-    const double inv = 1.0 / rijm;
-    const double nx = rx * inv, ny = ry * inv, nz = rz * inv;
-    const double Rst = ri * rj / (ri + rj);
-    const double meff = mi * mj / (mi + mj);
-    double c1 = sqrt(Rst * fabs(rsum - rijm));
-    const double Kn = 2.0 * p.hmE * c1;
-    const double Kt = 8.0 * p.hmG * c1;
-    const double wsx = rj * wj.x + ri * wi.x;
-    const double wsy = rj * wj.y + ri * wi.y;
-    const double wsz = rj * wj.z + ri * wi.z;
-    const double vx = vj.x - vi.x + (ny * wsz - nz * wsy);
-    const double vy = vj.y - vi.y + (nz * wsx - nx * wsz);
-    const double vz = vj.z - vi.z + (nx * wsy - ny * wsx);
-    const double vn = nx * vx + ny * vy + nz * vz;
-    const double fn = (rijm - rsum) * Kn * kTwoThirds;
-    const double fd = kDamp * p.pA * vn * sqrt(Kn * meff);
-    const double vtx = vx - nx * vn, vty = vy - ny * vn, vtz = vz - nz * vn;
-    const double o0 = oprev.x, o1 = oprev.y, o2 = oprev.z;
-    c1 = nx * o0 + ny * o1 + nz * o2;
-    double ox = o0 - nx * c1, oy = o1 - ny * c1, oz = o2 - nz * c1;
-    c1 = ox * ox + oy * oy + oz * oz;
-    if (c1 > 1e-18) {
-        c1 = 1.0 / c1;
-        c1 *= o0 * o0 + o1 * o1 + o2 * o2;
-        c1 = sqrt(c1);
-        ox *= c1; oy *= c1; oz *= c1;
-    }
-    ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
-    double ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
-    const double ftm = ftx * ftx + fty * fty + ftz * ftz;
-    if (ftm > p.mu * p.mu * fn * fn) {
-        c1 = p.mu * fabs(fn) * (1.0 / sqrt(ftm));
-        ftx *= c1; fty *= c1; ftz *= c1;
-        c1 = 1.0 / Kt;
-        ox = ftx * c1; oy = fty * c1; oz = ftz * c1;
-    } else {
-        c1 = kDamp * p.pA * sqrt(Kt * meff);
-        ftx += vtx * c1; fty += vty * c1; ftz += vtz * c1;
-    }
-    onew = make_double4(ox, oy, oz, 0.0);
-    c[0] = nx * (fn + fd) + ftx;
-    c[1] = ny * (fn + fd) + fty;
-    c[2] = nz * (fn + fd) + ftz;
-    const double tqx = (ny * ftz - nz * fty) * ri;
-    const double tqy = (nz * ftx - nx * ftz) * ri;
-    const double tqz = (nx * fty - ny * ftx) * ri;
-    if (roll) {
-        const double cr = -p.mur * fabs(fn) * (1.0 / sqrt(wim));
-        c[3] = tqx + wi.x * cr; c[4] = tqy + wi.y * cr; c[5] = tqz + wi.z * cr;
-    } else {
-        c[3] = tqx; c[4] = tqy; c[5] = tqz;
-    }
+    const double op[3] = {oprev.x, oprev.y, oprev.z};
+    double on[3];
+    contact_eval_t<double>(mat_of(p), p, v4(pi), v4(vi), v4(wi), v4(pj), v4(vj), v4(wj), op, wim,
+                           roll, 0.0, c, on);
+    onew = make_double4(on[0], on[1], on[2], 0.0);
 }
 
 // One 8-lane group per particle; lane l owns slots, candidates and bonds

@@ -101,6 +101,11 @@ struct rf_ctx {
     // A captured chunk of `graph_steps` (step, controller) launch pairs,
     // replayed while the constants it captured are unchanged.
     bool use_graph = true;
+    // sensitivity runs
+    bool sens_on = false;
+    int32_t sens_np = 0;
+    int64_t sens_curve0 = 0;
+    rf::SensState sens{};
     hipGraphExec_t graph_exec = nullptr;
     int64_t graph_steps = 0;
     rf::DevParams graph_p{};
@@ -1203,6 +1208,168 @@ rf_status rf_batch_run(rf_batch* b, rf_test_state* out) {
 rf_status rf_batch_last_ms(rf_batch* b, double* total_ms) {
     if (!b || !total_ms) return fail(RF_ERR_ARG, "null argument");
     *total_ms = b->last_ms;
+    return RF_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------- sensitivities ---
+// Forward-mode derivatives of the loading response with respect to material
+// parameters (rf_sens.hip): one run carries up to kSensK parameter
+// directions next to the values.
+
+namespace {
+
+// d(E, G, hm_E, hm_G, mu, mu_r, p_A, E_wall, G_wall, ratio, b_abs)/d(param),
+// differentiating rf_derive's formulas; the wall factor scales hm_E, hm_G.
+rf_status material_tangent(const rf_material* m, double wall_factor, int32_t param, double out[11]) {
+    for (int u = 0; u < 11; ++u) out[u] = 0.0;
+    const double nu = m->nu, Eb = m->E_bond, Ec = m->E_contact;
+    const double Gc = 0.5 * Ec / (1.0 + nu);
+    switch (param) {
+        case RF_PARAM_E_BOND:
+            out[0] = 1.0;
+            out[1] = 0.5 / (1.0 + nu);
+            break;
+        case RF_PARAM_NU: {
+            out[1] = -0.5 * Eb / ((1.0 + nu) * (1.0 + nu));
+            const double dGc = -0.5 * Ec / ((1.0 + nu) * (1.0 + nu));
+            out[2] = Ec * nu / ((1.0 - nu * nu) * (1.0 - nu * nu));
+            out[3] = 0.5 * (dGc * (2.0 - nu) + Gc) / ((2.0 - nu) * (2.0 - nu));
+            break;
+        }
+        case RF_PARAM_E_CONTACT:
+            out[2] = 0.5 / (1.0 - nu * nu);
+            out[3] = 0.5 * (0.5 / (1.0 + nu)) / (2.0 - nu);
+            break;
+        case RF_PARAM_FRICTION: out[4] = 1.0; break;
+        case RF_PARAM_ROLLING_FRICTION: out[5] = 1.0; break;
+        case RF_PARAM_BOND_RADIUS_RATIO:
+            if (!(m->bond_radius_ratio > 0.0))
+                return fail(RF_ERR_ARG, "bond radius ratio is not in use (absolute bond radius)");
+            out[9] = 1.0;
+            break;
+        default: return fail(RF_ERR_ARG, "unknown parameter %d", param);
+    }
+    out[7] = wall_factor * out[2];
+    out[8] = wall_factor * out[3];
+    return RF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_sens_material_tangent(const rf_material* m, double wall_factor, int32_t param,
+                                   double* out11) {
+    if (!m || !out11) return fail(RF_ERR_ARG, "null argument");
+    return material_tangent(m, wall_factor, param, out11);
+}
+
+rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
+    if (!c || !params || nparam < 1 || nparam > rf::kSensK)
+        return fail(RF_ERR_ARG, "rf_sens_begin: 1..%d parameters", rf::kSensK);
+    if (!c->built) return fail(RF_ERR_STATE, "rf_sens_begin before rf_build_bonds");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    rf::SensState& t = c->sens;
+    const int64_t n = c->n, K = rf::kSensK;
+    if (!c->sens_on) {
+        for (int b = 0; b < 2; ++b) {
+            rf_status st = c->alloc(&t.tpos[b], K * n);
+            if (!st) st = c->alloc(&t.tvel[b], K * n);
+            if (!st) st = c->alloc(&t.tang[b], K * n);
+            if (st) return st;
+        }
+        rf_status st = c->alloc(&t.tbrec, K * n * c->B * 12);
+        if (!st) st = c->alloc(&t.tot, K * n * c->S);
+        if (!st) st = c->alloc(&t.tpart_b, K * c->p.nblk);
+        if (!st) st = c->alloc(&t.tpart_t, K * c->p.nblk);
+        if (st) return st;
+        c->sens_on = true;
+    }
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipMemsetAsync(t.tpos[b], 0, K * n * sizeof(double4), c->stream));
+        HIP_TRY(hipMemsetAsync(t.tvel[b], 0, K * n * sizeof(double4), c->stream));
+        HIP_TRY(hipMemsetAsync(t.tang[b], 0, K * n * sizeof(double4), c->stream));
+    }
+    HIP_TRY(hipMemsetAsync(t.tbrec, 0, K * n * c->B * 12 * sizeof(double), c->stream));
+    HIP_TRY(hipMemsetAsync(t.tot, 0, K * n * c->S * sizeof(double4), c->stream));
+    const int64_t cap = std::max<int64_t>(c->p.curve_cap, 1);
+    if (t.tcurve) {
+        hipFree(t.tcurve);
+        c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void*)t.tcurve),
+                        c->allocs.end());
+        t.tcurve = nullptr;
+    }
+    rf_status st = c->alloc(&t.tcurve, cap * K * 2);
+    if (st) return st;
+    const double wf = c->has_test ? c->cfg.wall_modulus_factor : 0.0;
+    for (int k = 0; k < K; ++k) {
+        if (k < nparam) {
+            st = material_tangent(&c->mat, wf, params[k], t.dmat[k]);
+            if (st) return st;
+        } else {
+            for (int u = 0; u < 11; ++u) t.dmat[k][u] = 0.0;
+        }
+    }
+    c->sens_np = nparam;
+    c->sens_curve0 = c->hctrl.stepsave;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return RF_OK;
+}
+
+rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
+    if (done_steps) *done_steps = 0;
+    if (!c || !c->sens_on || c->sens_np < 1) return fail(RF_ERR_STATE, "rf_sens_step before rf_sens_begin");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    const int64_t start = c->hctrl.step;
+    int64_t remaining = nsteps;
+    while (remaining > 0 && !c->hctrl.done) {
+        const int64_t k = std::min<int64_t>(remaining, c->chunk);
+        const int64_t before = c->hctrl.step;
+        for (int64_t s = 0; s < k; ++s) {
+            HIP_TRY(rf::launch_sens_step(c->d, c->p, c->sens, c->ctrl, c->stream));
+            HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+        }
+        st = sync_ctrl(c);
+        if (st) return st;
+        const int64_t executed = c->hctrl.step - before;
+        remaining -= executed;
+        if (c->hctrl.err & rf::kErrSensList)
+            return fail(RF_ERR_STATE, "sensitivity window: a touching pair has no slot, two "
+                                      "particles coincide, or a particle left the box");
+        if (c->hctrl.err) return device_error(c);
+        if (c->hctrl.rebuild) {
+            st = rebuild(c);
+            if (st) return st;
+            if (c->hctrl.err) return device_error(c);
+        } else if (executed == 0 && !c->hctrl.done) {
+            return fail(RF_ERR_STATE, "sensitivity loop halted without a reason");
+        }
+    }
+    if (done_steps) *done_steps = c->hctrl.step - start;
+    return RF_OK;
+}
+
+rf_status rf_sens_get_curve(rf_ctx* c, double* out, int64_t cap, int64_t* n) {
+    if (n) *n = 0;
+    if (!c || !c->sens_on) return fail(RF_ERR_STATE, "no sensitivity run");
+    HIP_TRY(hipSetDevice(c->device));
+    rf_status st = sync_ctrl(c);
+    if (st) return st;
+    const int64_t m = std::min<int64_t>(c->hctrl.stepsave, c->p.curve_cap) - c->sens_curve0;
+    const int64_t cnt = std::max<int64_t>(0, std::min<int64_t>(m, cap));
+    std::vector<double> h((size_t)std::max<int64_t>(c->p.curve_cap, 1) * rf::kSensK * 2);
+    HIP_TRY(hipMemcpy(h.data(), c->sens.tcurve, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int np = c->sens_np;
+    for (int64_t q = 0; q < cnt; ++q)
+        for (int k = 0; k < np; ++k)
+            for (int e = 0; e < 2; ++e)
+                out[(q * np + k) * 2 + e] = h[((c->sens_curve0 + q) * rf::kSensK + k) * 2 + e];
+    if (n) *n = cnt;
     return RF_OK;
 }
 

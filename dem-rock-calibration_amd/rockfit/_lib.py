@@ -27,6 +27,7 @@ EXPORTS = [
     "rf_get_bond_counts", "rf_get_counters", "rf_get_state", "rf_set_state",
     "rf_num_particles", "rf_num_slots", "rf_set_timing", "rf_last_timing", "rf_set_chunk",
     "rf_fire_defaults", "rf_fire_relax", "rf_set_graph",
+    "rf_sens_material_tangent", "rf_sens_begin", "rf_sens_step", "rf_sens_get_curve",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
 ]
 
@@ -82,6 +83,14 @@ def lib():
     L.rf_set_timing.restype = st
     L.rf_last_timing.argtypes = [vp, P(C.c_double), P(C.c_int64), P(C.c_double), P(C.c_int64)]
     L.rf_last_timing.restype = st
+    L.rf_sens_material_tangent.argtypes = [P(abi.Material), C.c_double, C.c_int32, P(C.c_double)]
+    L.rf_sens_material_tangent.restype = st
+    L.rf_sens_begin.argtypes = [vp, C.c_int32, P(C.c_int32)]
+    L.rf_sens_begin.restype = st
+    L.rf_sens_step.argtypes = [vp, C.c_int64, P(C.c_int64)]
+    L.rf_sens_step.restype = st
+    L.rf_sens_get_curve.argtypes = [vp, P(C.c_double), C.c_int64, P(C.c_int64)]
+    L.rf_sens_get_curve.restype = st
     L.rf_set_graph.argtypes = [vp, C.c_int32]
     L.rf_set_graph.restype = st
     L.rf_batch_create.argtypes = [P(vp), C.c_int32, P(vp)]

@@ -110,6 +110,15 @@ class StateView(C.Structure):
 
 
 RF_QUIRK_FIRE_NO_INERTIA = 0x8
+
+# rf_param
+RF_PARAM_E_BOND = 0
+RF_PARAM_NU = 1
+RF_PARAM_E_CONTACT = 2
+RF_PARAM_FRICTION = 3
+RF_PARAM_ROLLING_FRICTION = 4
+RF_PARAM_BOND_RADIUS_RATIO = 5
+RF_SENS_MAX = 4
 RF_CONTAINER_NONE = 0
 RF_CONTAINER_CYLINDER_Z = 1
 

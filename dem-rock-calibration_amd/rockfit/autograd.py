@@ -1,0 +1,85 @@
+"""The loading response as a differentiable function of material
+parameters: torch.autograd.Function over the GPU sensitivity run.
+
+forward(theta) runs one loading window of a fixed sample on the GPU with
+the parameter directions seeded (rf_sens_*), returns the platen stresses at
+the curve points (MPa, bottom and top) and keeps the Jacobian
+d(stress)/d(theta) that the same run produced; backward returns J^T g.
+This replaces the finite-difference Jacobian of the reference's calibration
+(CalculateStepCG, mp_determineconjgradient.cu:84-142).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import abi
+from .core import Context, derive
+
+# name -> (rf_param, Material field); E in reduced units (GPa)
+PARAMS = {
+    "E_bond": (abi.RF_PARAM_E_BOND, "E_bond"),
+    "nu": (abi.RF_PARAM_NU, "nu"),
+    "E_contact": (abi.RF_PARAM_E_CONTACT, "E_contact"),
+    "friction": (abi.RF_PARAM_FRICTION, "friction"),
+    "rolling_friction": (abi.RF_PARAM_ROLLING_FRICTION, "rolling_friction"),
+    "bond_radius_ratio": (abi.RF_PARAM_BOND_RADIUS_RATIO, "bond_radius_ratio"),
+}
+
+
+def with_params(mat, names, values):
+    """Copy of `mat` with the named fields set; dt fixed to the base value
+    (the step size is a constant of a sensitivity run)."""
+    m = abi.Material.from_buffer_copy(mat)
+    if not (m.dt > 0.0):
+        raise ValueError("fix dt first (fixed_dt): the step size must not follow the parameters")
+    for nme, v in zip(names, values):
+        setattr(m, PARAMS[nme][1], float(v))
+    return m
+
+
+def fixed_dt(mat, r):
+    """Material with dt frozen at the value rf_derive gives for it."""
+    m = abi.Material.from_buffer_copy(mat)
+    if not (m.dt > 0.0):
+        m.dt = derive(mat, float(np.min(r)), float(np.max(r))).dt
+    return m
+
+
+def run_window(setup, names, values, sens=True):
+    """One loading window: returns (stress (points, 2) reduced units,
+    Jacobian (points, nparam, 2) or None)."""
+    mat = with_params(setup["mat"], names, values)
+    c = Context(setup["xyz"], setup["r"], mat, setup["box"])
+    c.build_bonds()
+    c.set_test(setup["cfg"])
+    if sens:
+        c.sens_begin([PARAMS[n][0] for n in names])
+        done = c.sens_step(setup["steps"])
+    else:
+        done = c.step(setup["steps"])
+    if done != setup["steps"]:
+        raise RuntimeError(f"the loading window ended after {done} of {setup['steps']} steps")
+    curve = c.curve()
+    J = c.sens_curve() if sens else None
+    return curve[:, 1:3].copy(), J
+
+
+class LoadingResponse(torch.autograd.Function):
+    """stress[MPa] (points, 2) = f(theta); setup = dict(xyz, r, mat (dt > 0),
+    box, cfg, steps, names)."""
+
+    @staticmethod
+    def forward(ctx, theta, setup):
+        values = theta.detach().cpu().numpy().astype(np.float64)
+        stress, J = run_window(setup, setup["names"], values, sens=True)
+        if J.shape[0] != stress.shape[0]:
+            raise RuntimeError("curve and curve tangents differ in length")
+        ctx.save_for_backward(torch.from_numpy(J * 1e3))
+        return torch.from_numpy(stress * 1e3).to(theta.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        (J,) = ctx.saved_tensors
+        grad = torch.einsum("pe,pke->k", g.to(J.dtype), J)
+        return grad, None

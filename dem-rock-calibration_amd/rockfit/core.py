@@ -143,6 +143,27 @@ class Context:
         """Bracket every `every`-th step-kernel launch with HIP events (0: off)."""
         check(lib().rf_set_timing(self.h, int(every)), "rf_set_timing")
 
+    def sens_begin(self, params):
+        """Start a sensitivity window along the given rf_param directions."""
+        arr = (C.c_int32 * len(params))(*[int(q) for q in params])
+        check(lib().rf_sens_begin(self.h, len(params), arr), "rf_sens_begin")
+        self._sens_np = len(params)
+
+    def sens_step(self, nsteps):
+        done = C.c_int64()
+        check(lib().rf_sens_step(self.h, int(nsteps), C.byref(done)), "rf_sens_step")
+        return done.value
+
+    def sens_curve(self):
+        """(points, nparam, 2): d(bottom, top platen stress)/d(param) at the
+        curve points recorded since sens_begin (reduced stress units)."""
+        cap = 1 << 20
+        buf = np.zeros(cap * self._sens_np * 2)
+        n = C.c_int64()
+        check(lib().rf_sens_get_curve(self.h, buf.ctypes.data_as(C.POINTER(C.c_double)), cap,
+                                      C.byref(n)), "rf_sens_get_curve")
+        return buf[: n.value * self._sens_np * 2].reshape(n.value, self._sens_np, 2)
+
     def set_graph(self, on=True):
         """Replay full rf_step chunks as captured HIP graphs (default on)."""
         check(lib().rf_set_graph(self.h, 1 if on else 0), "rf_set_graph")
@@ -157,6 +178,15 @@ class Context:
         check(lib().rf_last_timing(self.h, C.byref(k), C.byref(n), C.byref(t), C.byref(s)),
               "rf_last_timing")
         return k.value, n.value, t.value, s.value
+
+
+def material_tangent(mat, param, wall_modulus_factor=0.0):
+    """Derivatives of (E, G, hm_E, hm_G, mu, mu_r, p_A, E_wall, G_wall,
+    ratio, bond radius) with respect to one rf_param."""
+    out = (C.c_double * 11)()
+    check(lib().rf_sens_material_tangent(C.byref(mat), float(wall_modulus_factor), int(param),
+                                          out), "rf_sens_material_tangent")
+    return np.array(out[:])
 
 
 class Batch:

@@ -245,6 +245,42 @@ rf_status rf_set_chunk(rf_ctx* ctx, int32_t steps_per_sync);
 rf_status rf_set_graph(rf_ctx* ctx, int32_t enable);
 
 /* ------------------------------------------------------------------------
+ * Sensitivities: forward-mode (tangent-linear) derivatives of the loading
+ * response with respect to material parameters, up to RF_SENS_MAX per run.
+ * Replaces the finite-difference Jacobian of the reference's calibration
+ * (CalculateStepCG, mp_determineconjgradient.cu:84-142, NParam+1 forward
+ * runs of CalculateOneSample) by one run carrying the parameter directions.
+ * Between rf_sens_begin and the last rf_sens_step the context advances with
+ * the sensitivity kernel (values equal the plain step's bit for bit: same
+ * operations, same per-particle sum order); the slot table is held
+ * (pairs crossing acut carry no force; a touching pair without a slot ends
+ * the window with RF_ERR_STATE), dt is a constant of the run.
+ * ------------------------------------------------------------------------ */
+#define RF_SENS_MAX 4
+
+typedef enum rf_param {
+    RF_PARAM_E_BOND = 0,             /* bond Young's modulus (G follows via nu) */
+    RF_PARAM_NU = 1,                 /* Poisson ratio (bond G, contact hm_E/hm_G) */
+    RF_PARAM_E_CONTACT = 2,          /* contact Young's modulus               */
+    RF_PARAM_FRICTION = 3,           /* sliding friction mu                   */
+    RF_PARAM_ROLLING_FRICTION = 4,   /* rolling friction mu_r                 */
+    RF_PARAM_BOND_RADIUS_RATIO = 5   /* bond radius / min(r_i, r_j)           */
+} rf_param;
+
+/* Derivatives of the material constants (E, G, hm_E, hm_G, mu, mu_r, p_A,
+ * E_wall, G_wall, ratio, bond radius) with respect to one parameter. */
+rf_status rf_sens_material_tangent(const rf_material* mat, double wall_modulus_factor,
+                                   int32_t param, double* out11);
+/* Zero the tangents and seed the parameter directions (params: nparam
+ * rf_param values).  The curve tangents start at the next curve point. */
+rf_status rf_sens_begin(rf_ctx* ctx, int32_t nparam, const int32_t* params);
+/* Advance values and tangents by up to nsteps loading steps. */
+rf_status rf_sens_step(rf_ctx* ctx, int64_t nsteps, int64_t* done_steps);
+/* Tangents of the curve points recorded since rf_sens_begin:
+ * out[(point*nparam + k)*2 + {0,1}] = d(bottom, top platen stress)/d(param k). */
+rf_status rf_sens_get_curve(rf_ctx* ctx, double* out, int64_t cap, int64_t* n);
+
+/* ------------------------------------------------------------------------
  * Batches: several samples advanced together, one step launch and one
  * controller launch per step for all of them.  No GPU counterpart in the
  * reference: it replaces running samples one after another

@@ -1,0 +1,114 @@
+"""GPU sensitivities (rf_sens_*, rockfit.autograd) against central finite
+differences of the plain GPU step.
+
+Loading window of a bonded 2000-sphere sample: the platens close in, touch
+and load the specimen to about 3 MPa before the first bond breaks.
+
+What finite differences can check: on a short window (30 steps) central
+differences converge for relative steps 1e-4 ... 1e-6, and are the
+reference; on longer windows the stick/slip and rolling-friction switches of
+the contact law make small-step differences erratic (they scatter by orders
+of magnitude between 1e-5 and 1e-7), while a 3 % step gives the smooth
+response's slope.  The tangents are exact except for the direction of a
+rolling-friction torque in its chatter regime (rf_physics.h), which costs
+about 1 %.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import gpu_available
+from rockfit import autograd, config
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["E_bond", "nu", "E_contact", "friction"]
+
+
+@pytest.fixture(autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.skip("no GPU")
+
+
+@pytest.fixture(scope="module")
+def base():
+    w = config.make_workload(n=2000, r_min=0.2, r_max=0.3, test="uniaxial", seed=21,
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=10, max_steps=5000,
+                                              max_strain=0.05))
+    mat = autograd.fixed_dt(w["mat"], w["r"])
+    return dict(xyz=w["xyz"], r=w["r"], mat=mat, box=w["box"], cfg=w["cfg"], names=NAMES)
+
+
+def window(base, steps):
+    return dict(base, steps=steps)
+
+
+def theta0(setup):
+    m = setup["mat"]
+    return np.array([getattr(m, autograd.PARAMS[n][1]) for n in NAMES])
+
+
+def central(setup, k, rel):
+    th = theta0(setup)
+    h = rel * abs(th[k])
+    tp, tm = th.copy(), th.copy()
+    tp[k] += h
+    tm[k] -= h
+    sp, _ = autograd.run_window(setup, NAMES, tp, sens=False)
+    sm, _ = autograd.run_window(setup, NAMES, tm, sens=False)
+    return (sp - sm) / (2 * h)
+
+
+def check_against_fd(setup, rel, tol):
+    print(f"case window {setup['steps']} rel {rel}", flush=True)
+    _, J = autograd.run_window(setup, NAMES, theta0(setup), sens=True)
+    fds = [central(setup, k, rel) for k in range(len(NAMES))]
+    scale_E = np.abs(fds[0]).max()
+    assert scale_E > 0.0
+    for k, name in enumerate(NAMES):
+        # friction acts through stick/slip switches: a secant of a few percent
+        # crosses them, so its slope is compared on the scale of the main
+        # (bond modulus) sensitivity
+        floor = 0.1 if name != "friction" else 1.0
+        ref = max(np.abs(fds[k]).max(), floor * scale_E)
+        err = np.abs(J[:, k, :] - fds[k]).max()
+        assert err <= tol * ref, (name, err, ref)
+
+
+def test_sens_values_equal_the_plain_step(base):
+    setup = window(base, 300)
+    s_sens, _ = autograd.run_window(setup, NAMES, theta0(setup), sens=True)
+    s_plain, _ = autograd.run_window(setup, NAMES, theta0(setup), sens=False)
+    assert np.abs(s_plain).max() > 0.0, "platens never loaded the specimen"
+    np.testing.assert_array_equal(s_sens, s_plain)
+
+
+def test_sens_short_window_against_converged_differences(base):
+    check_against_fd(window(base, 30), 1e-5, 0.02)
+
+
+def test_sens_long_window_against_macroscopic_differences(base):
+    check_against_fd(window(base, 300), 3e-2, 0.05)
+
+
+def test_directions_are_independent(base):
+    setup = window(base, 300)
+    th = theta0(setup)
+    _, J4 = autograd.run_window(setup, NAMES, th, sens=True)
+    _, J1 = autograd.run_window(setup, NAMES[:1], th[:1], sens=True)
+    np.testing.assert_array_equal(J4[:, 0, :], J1[:, 0, :])
+
+
+def test_autograd_function(base):
+    setup = window(base, 300)
+    th = torch.tensor(theta0(setup), dtype=torch.float64, requires_grad=True)
+    stress = autograd.LoadingResponse.apply(th, setup)
+    w = torch.linspace(0.5, 1.5, stress.shape[0], dtype=torch.float64)[:, None]
+    loss = (stress * w).sum()
+    loss.backward()
+    _, J = autograd.run_window(setup, NAMES, theta0(setup), sens=True)
+    expect = np.einsum("pe,pke->k", np.broadcast_to(w.numpy(), stress.shape), J * 1e3)
+    np.testing.assert_allclose(th.grad.numpy(), expect, rtol=1e-13, atol=0.0)  # J^T g
+    fd = (central(setup, 0, 3e-2) * 1e3 * w.numpy()).sum()
+    assert abs(th.grad[0].item() - fd) <= 0.02 * abs(fd)
