@@ -42,7 +42,7 @@ struct FireParams {
     int32_t nblk, K;
     double D, dt0, dtmax, dtmin, grow, shrink, ashrink, alpha0;
     int64_t ndelay, npmax, max_steps, cap;
-    int32_t container, noinertia, check_every, delete_after;
+    int32_t container, noinertia, check_every, delete_after, law, pad_;
     double cen[3], R0, H0, hide_z, hidden[3], gap_tol;
     double list_r2, skin_half2;
     double lo[3], inv_cell;
@@ -199,8 +199,14 @@ __global__ __launch_bounds__(kFB) void k_fire_force(FireDev d, FireParams p, Fir
             const double a = pi.w + pj.w;
             const double aa = a * a;
             if (rr < aa) {
-                const double q1 = aa * (1.0 / rr);
-                const double fm = -p.D * q1;
+                double fm;
+                if (p.law == RF_FIRE_LAW_HARMONIC) {
+                    const double rho = sqrt(rr);
+                    fm = -p.D * ((a - rho) / rho);
+                } else {
+                    const double q1 = aa * (1.0 / rr);
+                    fm = -p.D * q1;
+                }
                 fx += fm * dx;
                 fy += fm * dy;
                 fz += fm * dz;
@@ -363,7 +369,8 @@ __global__ __launch_bounds__(kFB) void k_fire_move(FireDev d, FireParams p, Fire
     r.x += vx * dt;
     r.y += vy * dt;
     r.z += vz * dt;
-    if (p.container == RF_CONTAINER_CYLINDER_Z) {
+    if (p.container == RF_CONTAINER_CYLINDER_Z || p.container == RF_CONTAINER_CYLINDER_Z_ABSORB) {
+        const bool absorb = p.container == RF_CONTAINER_CYLINDER_Z_ABSORB;
         const double ex = r.x - p.cen[0], ey = r.y - p.cen[1], ez = r.z - p.cen[2];
         double sx = 0.0, sy = 0.0, sz = 0.0;
         const double rr = ex * ex + ey * ey;
@@ -372,16 +379,25 @@ __global__ __launch_bounds__(kFB) void k_fire_move(FireDev d, FireParams p, Fire
                 const double q = 1.0 / sqrt(rr);
                 sx = ex * (p.R0 * q - 1.0);
                 sy = ey * (p.R0 * q - 1.0);
-                vx *= -1.0;
-                vy *= -1.0;
+                if (absorb) {
+                    const double ux = ex * q, uy = ey * q;
+                    const double vr = vx * ux + vy * uy;
+                    if (vr > 0.0) {
+                        vx -= vr * ux;
+                        vy -= vr * uy;
+                    }
+                } else {
+                    vx *= -1.0;
+                    vy *= -1.0;
+                }
             }
             if (ez > p.H0) {
                 sz = p.H0 - ez;
-                vz *= -1.0;
+                vz = absorb ? (vz > 0.0 ? 0.0 : vz) : -vz;
             }
             if (ez < -p.H0) {
                 sz = -p.H0 - ez;
-                vz *= -1.0;
+                vz = absorb ? (vz < 0.0 ? 0.0 : vz) : -vz;
             }
             r.x += sx;
             r.y += sy;
@@ -470,6 +486,10 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
     if (n <= 0 || !x || !y || !z || !r || !cfg || !out)
         return rf::report(RF_ERR_ARG, "rf_fire_relax: bad arguments");
     if (n >= (int64_t)1 << 31) return rf::report(RF_ERR_ARG, "rf_fire_relax: n too large");
+    if (cfg->law != RF_FIRE_LAW_REFERENCE && cfg->law != RF_FIRE_LAW_HARMONIC)
+        return rf::report(RF_ERR_ARG, "rf_fire_relax: unknown force law %d", (int)cfg->law);
+    if (cfg->container < RF_CONTAINER_NONE || cfg->container > RF_CONTAINER_CYLINDER_Z_ABSORB)
+        return rf::report(RF_ERR_ARG, "rf_fire_relax: unknown container %d", (int)cfg->container);
     if (!(cfg->dt0 > 0.0) || !(cfg->density > 0.0) || cfg->max_steps < 0)
         return rf::report(RF_ERR_ARG, "rf_fire_relax: dt0, density must be > 0, max_steps >= 0");
     int ndev = 0;
@@ -489,7 +509,7 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
             hi[k] = std::max(hi[k], q[k]);
         }
     }
-    if (cfg->container == RF_CONTAINER_CYLINDER_Z) {
+    if (cfg->container == RF_CONTAINER_CYLINDER_Z || cfg->container == RF_CONTAINER_CYLINDER_Z_ABSORB) {
         const double ext[3] = {cfg->radius, cfg->radius, cfg->half_height};
         for (int k = 0; k < 3; ++k) {
             lo[k] = std::min(lo[k], cfg->center[k] - ext[k]);
@@ -517,6 +537,7 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
     p.cap = cfg->hard_cap > 0 ? cfg->hard_cap : 10 * cfg->max_steps;
     p.container = cfg->container;
     p.noinertia = (cfg->quirks & RF_QUIRK_FIRE_NO_INERTIA) ? 1 : 0;
+    p.law = cfg->law;
     p.check_every = cfg->check_every;
     p.delete_after = cfg->delete_after;
     for (int k = 0; k < 3; ++k) {

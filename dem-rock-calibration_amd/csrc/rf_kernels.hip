@@ -575,8 +575,11 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         s.bnd[i] = bnd;
         s.otnz[i] = otnz;
         if (nc_out != nc) s.ncand[i] = nc_out;
+        // a particle that has just been moved to the hidden point changes the
+        // lists; one that sits there already (and stays: F = V = 0) does not
+        const bool was_hidden = (pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]);
         const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
-        if (out || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
+        if ((out & !was_hidden) || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
         sh_b[grp] = part_b;
         sh_t[grp] = part_t;
         sh_nb[grp] = __popc(bnd);
@@ -907,13 +910,16 @@ __global__ void k_prune_initial(DevState s, DevParams p, Ctrl* ctrl) {
     s.otnz[i] &= ~drop;
 }
 
-// Candidate lists: every j within acut + skin (27 cells of edge 2 acut), plus
-// every current row partner; cslot links a candidate to its slot.
+// Candidate lists: every j within acut + skin (27 cells of edge 2 acut) that
+// is not a current row partner.  Particles at the hidden point neither have
+// nor are candidates: they sit at one point outside the box (rho = 0 never
+// links, d_ReConstructInteractionListbpm's 1e-10 < rr) and never move again.
 __global__ void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     const double4* pos = s.pos[ctrl->cur];
     const double4 pi = pos[i];
+    const bool hidden_i = (pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]);
     const int64_t key = s.ckey[i];
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t C = p.C;
@@ -925,10 +931,11 @@ __global__ void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
             for (int dx = -1; dx <= 1; ++dx) {
                 const int64_t cc = key + dx + dy * nx + dz * nxy;
                 const int32_t st = s.cstart[cc], ct = s.ccount[cc];
-                for (int32_t q = 0; q < ct; ++q) {
+                for (int32_t q = 0; q < ct && !hidden_i; ++q) {
                     const int32_t j = s.sorted[st + q];
                     if (j == (int32_t)i) continue;
                     const double4 pj = pos[j];
+                    if ((pj.x == p.hid[0]) & (pj.y == p.hid[1]) & (pj.z == p.hid[2])) continue;
                     const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
                     if (!(ex * ex + ey * ey + ez * ez < p.cand_r2)) continue;
                     bool partner = false;  // current partners hold a slot already

@@ -79,7 +79,8 @@ struct Buf {
 struct rf_ctx {
     int64_t n = 0;
     int32_t S = 32, C = 32, B = 32;
-    double skin = 0.0;  // current candidate-list skin
+    double skin = 0.0;      // current candidate-list skin
+    double skin_set = 0.0;  // the skin asked for (rf_options.skin or its default)
     rf_material mat{};
     rf_derived der{};
     rf_box box{};
@@ -227,6 +228,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     c->B = rf::kMaxSlots;
     c->mat = *mat;
     c->skin = o.skin;
+    c->skin_set = o.skin;
     c->box = *box;
     c->opt = o;
     c->device = o.device;
@@ -273,7 +275,14 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     ALLOC(d.cand, nC);
     ALLOC(d.cslot, nC);
     ALLOC(d.ncand, n);
-    const int64_t ncell = (int64_t)box->dims[0] * box->dims[1] * box->dims[2];
+    // Working grid: the reference's dims plus one cell layer beyond the top
+    // x and y faces.  The reference grid ends inside the last cell that holds
+    // the space box (Nr = floor(B/a) + 1), so a particle near B would need a
+    // stencil cell outside it; with the extra layer every point of the space
+    // box (and the hidden point) lies in cells 1..dims-2.  Cell membership,
+    // and so every list order, is unchanged.
+    const int32_t gx = box->dims[0] + 1, gy = box->dims[1] + 1, gz = box->dims[2];
+    const int64_t ncell = (int64_t)gx * gy * gz;
     ALLOC(d.ckey, n);
     ALLOC(d.ccount, ncell);
     ALLOC(d.cstart, ncell);
@@ -334,9 +343,9 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     p.b_abs = mat->bond_radius;
     p.bond_gap = mat->bond_gap;
     p.quirks = mat->quirks;
-    p.nx = box->dims[0];
-    p.ny = box->dims[1];
-    p.nz = box->dims[2];
+    p.nx = gx;
+    p.ny = gy;
+    p.nz = gz;
     p.ncell = ncell;
     for (int k = 0; k < 3; ++k) {
         p.cen[k] = box->center[k];
@@ -422,11 +431,20 @@ rf_status graph_chunk(rf_ctx* c, int64_t k) {
     return RF_OK;
 }
 
+void set_skin(rf_ctx* c, double skin) {
+    c->skin = skin;
+    c->p.cand_r2 = (c->mat.acut + c->skin) * (c->mat.acut + c->skin);
+    c->p.skin_half2 = 0.25 * c->skin * c->skin;
+}
+
 // Cell sort of the current positions and fresh candidate lists.  A list
 // longer than the candidate capacity halves the skin and builds again (the
 // skin only decides how often lists are rebuilt, never a result); below
-// acut/64 the overflow is an error.
+// acut/64 the overflow is an error.  A narrowed skin is doubled back toward
+// its set value at the next rebuild, so one crowded moment does not leave
+// the lists rebuilding every few steps for the rest of the run.
 rf_status rebuild(rf_ctx* c) {
+    if (c->skin < c->skin_set) set_skin(c, std::min(c->skin_set, 2.0 * c->skin));
     for (;;) {
         HIP_TRY(rf::launch_cell_keys(c->d, c->p, c->ctrl, c->stream));
         HIP_TRY(rf::launch_cell_sort(c->d, c->p, c->ctrl, c->stream));
@@ -436,9 +454,7 @@ rf_status rebuild(rf_ctx* c) {
         rf_status st = sync_ctrl(c);
         if (st) return st;
         if (c->hctrl.err != rf::kErrCandOverflow || c->skin < c->mat.acut / 64.0) return RF_OK;
-        c->skin *= 0.5;
-        c->p.cand_r2 = (c->mat.acut + c->skin) * (c->mat.acut + c->skin);
-        c->p.skin_half2 = 0.25 * c->skin * c->skin;
+        set_skin(c, 0.5 * c->skin);
         const int32_t zero = 0;
         HIP_TRY(hipMemcpyAsync(&c->ctrl->err, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(rf::launch_clear_flags(c->ctrl, c->stream));

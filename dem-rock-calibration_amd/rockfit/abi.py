@@ -121,6 +121,9 @@ RF_PARAM_BOND_RADIUS_RATIO = 5
 RF_SENS_MAX = 4
 RF_CONTAINER_NONE = 0
 RF_CONTAINER_CYLINDER_Z = 1
+RF_CONTAINER_CYLINDER_Z_ABSORB = 2
+RF_FIRE_LAW_REFERENCE = 0
+RF_FIRE_LAW_HARMONIC = 1
 
 
 class FireCfg(C.Structure):
@@ -133,7 +136,7 @@ class FireCfg(C.Structure):
         ("center", D3), ("radius", C.c_double), ("half_height", C.c_double),
         ("hide_z", C.c_double), ("hidden", D3), ("gap_tol", C.c_double),
         ("check_every", C.c_int32), ("delete_after", C.c_int32), ("skin", C.c_double),
-        ("quirks", C.c_uint32), ("device", C.c_int32),
+        ("quirks", C.c_uint32), ("device", C.c_int32), ("law", C.c_int32), ("pad_", C.c_int32),
     ]
 
 

@@ -146,9 +146,10 @@ def uniaxial_cfg(xyz, r, v_load_mps=-0.1, min_stress_mpa=10.0, max_strain=0.01,
                       cond_fracture, dnsave, 2.0, max_steps, curve_cap)
 
 
-def brazilian_cfg(xyz, r, v_load_mps=-0.02, min_stress_mpa=2.1, max_strain=0.03,
+def brazilian_cfg(xyz, r, v_load_mps=-0.1, min_stress_mpa=2.1, max_strain=0.03,
                   cond_fracture=0.1, dnsave=1000, max_steps=0, curve_cap=0):
-    """Brazilian driver values (mp_calculateonesample.cu:121-124, dNSaveB = 1000);
+    """Brazilian driver values (mp_calculateonesample.cu:121-124: BVelocity =
+    -5 * 0.02 m/s, dNSaveB = 1000);
     wall moduli hm_E, hm_G (md_calculationbraziltest.cu:198)."""
     return platen_cfg(xyz, r, abi.RF_TEST_BRAZILIAN, v_load_mps, min_stress_mpa, max_strain,
                       cond_fracture, dnsave, 1.0, max_steps, curve_cap)

@@ -75,8 +75,10 @@ typedef struct rf_material {
 /* Space box, hidden point and cell grid (sample_data / cell_data,
  * md_data_types.h:24-124).  Particles leaving center +- half are moved to the
  * hidden point (md_parallelepipedrestriction.cu:73-160).  The cell grid has
- * its origin at 0, edge `cell` and dims[0..2] cells; every particle must lie in
- * cells 1..dims-2 on each axis. */
+ * its origin at 0, edge `cell` and dims[0..2] cells (the reference's Nr).  The
+ * library works on that grid plus one cell layer past the top x and y faces,
+ * so a particle must lie in cells 1..dims[0]-1, 1..dims[1]-1 and
+ * 1..dims[2]-2, which hold the whole space box and the hidden point. */
 typedef struct rf_box {
     double center[3];
     double half[3];
@@ -318,8 +320,27 @@ rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
 
 typedef enum rf_container {
     RF_CONTAINER_NONE = 0,
-    RF_CONTAINER_CYLINDER_Z = 1  /* centres kept within radius and +-half_height of center */
+    RF_CONTAINER_CYLINDER_Z = 1,        /* centres kept within radius and +-half_height of
+                                           center; the velocity of a particle put back is
+                                           reversed (d_CylinderRestrictionZ)             */
+    RF_CONTAINER_CYLINDER_Z_ABSORB = 2  /* same bounds; only the outward velocity component
+                                           is removed (no reference counterpart: reversed
+                                           wall velocities keep the power F.V negative and
+                                           stall FIRE when a packing is grown under
+                                           pressure)                                      */
 } rf_container;
+
+/* Pair force of overlapping spheres (rho < a = r_i + r_j), on i:
+ *   REFERENCE: -D (a^2 / rho^2) (R_j - R_i), magnitude D a^2 / rho
+ *              (d_CalculateForces, md_calculateforces.cu:20-67);
+ *   HARMONIC:  -D ((a - rho) / rho) (R_j - R_i), magnitude D (a - rho).
+ * The harmonic law has no reference counterpart: its force vanishes at
+ * contact, so FIRE converges on dense packings where the reference law
+ * (finite force at contact) makes the power change sign every few steps. */
+typedef enum rf_fire_law {
+    RF_FIRE_LAW_REFERENCE = 0,
+    RF_FIRE_LAW_HARMONIC = 1
+} rf_fire_law;
 
 typedef struct rf_fire_cfg {
     double D;                 /* repulsion strength (reduced force per mm)      */
@@ -343,6 +364,8 @@ typedef struct rf_fire_cfg {
     double skin;              /* neighbour-list skin (GPU only; never changes results) */
     uint32_t quirks;          /* RF_QUIRK_FIRE_NO_INERTIA                       */
     int32_t device;
+    int32_t law;              /* rf_fire_law                                    */
+    int32_t pad_;
 } rf_fire_cfg;
 
 typedef struct rf_fire_result {

@@ -198,6 +198,12 @@ ora_ctx* ora_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     c->nthreads = nthreads > 0 ? nthreads : 1;
     c->mat = *mat;
     c->box = *box;
+    /* working grid: one more cell layer beyond the top x and y faces, so the
+     * whole space box lies in cells 1..dims-2 (the reference's last cell
+     * column can need a stencil cell past its grid); membership, hence every
+     * list order, is unchanged */
+    c->box.dims[0] += 1;
+    c->box.dims[1] += 1;
     size_t n3 = (size_t)(3 * n) * sizeof(double);
     c->R = zalloc(n3); c->V = zalloc(n3); c->W = zalloc(n3); c->F = zalloc(n3); c->M = zalloc(n3);
     c->rad = zalloc(n * sizeof(double)); c->mass = zalloc(n * sizeof(double));
@@ -213,7 +219,7 @@ ora_ctx* ora_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     c->pn = zalloc(3 * ns * sizeof(double)); c->ot = zalloc(3 * ns * sizeof(double));
     c->mn = zalloc(3 * ns * sizeof(double)); c->mt = zalloc(3 * ns * sizeof(double));
     c->key = zalloc(n * sizeof(int32_t)); c->sorted = zalloc(n * sizeof(int32_t));
-    c->ncell = (int64_t)box->dims[0] * box->dims[1] * box->dims[2];
+    c->ncell = (int64_t)c->box.dims[0] * c->box.dims[1] * c->box.dims[2];
     c->cstart = zalloc(c->ncell * sizeof(int32_t)); c->ccount = zalloc(c->ncell * sizeof(int32_t));
     c->part_b = zalloc(n * sizeof(double)); c->part_t = zalloc(n * sizeof(double));
     c->nblk = (n + RED_BLOCK - 1) / RED_BLOCK;
@@ -260,9 +266,10 @@ void ora_destroy(ora_ctx* c) {
  * (cub::DeviceRadixSort::SortPairs over identity ids, md_cell_distribution.cu:87)
  * and cell start/count (d_DetermineCellPointer, md_calculate_cell.cu:43-75).
  * Pinned by: stable counting sort equals any stable sort by key; tests check
- * against numpy's stable argsort.  Particles must lie in cells 1..dims-2 so
- * the 8-cell stencil stays inside the grid (the reference reads garbage
- * otherwise). */
+ * against numpy's stable argsort.  Particles must lie in cells 1..dims-2 of
+ * the working grid (the reference's plus one layer past the top x and y
+ * faces) so the 8-cell stencil stays inside it; the reference reads past its
+ * grid for particles in the upper half of its last x or y cell. */
 static int cell_coords(const ora_ctx* c, double x, double y, double z, int32_t* ix, int32_t* iy,
                        int32_t* iz) {
     double fx = floor(x * c->inv_a), fy = floor(y * c->inv_a), fz = floor(z * c->inv_a);

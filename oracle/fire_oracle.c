@@ -127,7 +127,7 @@ static int cmp_i64(const void* a, const void* b) {
 }
 
 /* d_CalculateForces for every live particle, plus its overlap gap. */
-static int forces(fire_state* s, double D, int nthreads) {
+static int forces(fire_state* s, double D, int law, int nthreads) {
     if (build_cells(s)) return -1;
     int err = 0;
 #pragma omp parallel num_threads(nthreads)
@@ -169,8 +169,14 @@ static int forces(fire_state* s, double D, int nthreads) {
                 const double a = s->r[i] + s->r[j];
                 const double aa = a * a;
                 if (rr < aa) {
-                    const double q1 = aa * (1.0 / rr);
-                    const double fm = -D * q1;
+                    double fm;
+                    if (law == RF_FIRE_LAW_HARMONIC) { /* build option, see rockfit_dem.h */
+                        const double rho = sqrt(rr);
+                        fm = -D * ((a - rho) / rho);
+                    } else {
+                        const double q1 = aa * (1.0 / rr);
+                        fm = -D * q1;
+                    }
                     fsx += fm * dx;
                     fsy += fm * dy;
                     fsz += fm * dz;
@@ -241,7 +247,7 @@ int ora_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
     while (1) {
         if (!(step < cfg->max_steps || min_gap < cfg->gap_tol)) { reason = 1; break; }
         if (step >= cap) { reason = 3; break; }
-        if (forces(&s, cfg->D, nthreads)) { rc = -1; break; }
+        if (forces(&s, cfg->D, cfg->law, nthreads)) { rc = -1; break; }
         P = power_sum(&s);
         int neg = 0;
         if (P > 0) {
@@ -286,7 +292,11 @@ int ora_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
             vz = (1.0 - alpha) * vz + vdf * Fz;
             s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
             x[i] += vx * dt; y[i] += vy * dt; z[i] += vz * dt;
-            if (cfg->container == RF_CONTAINER_CYLINDER_Z) {
+            if (cfg->container == RF_CONTAINER_CYLINDER_Z ||
+                cfg->container == RF_CONTAINER_CYLINDER_Z_ABSORB) {
+                /* absorbing variant (a build option, no reference counterpart):
+                 * remove the outward normal velocity instead of reversing */
+                const int absorb = cfg->container == RF_CONTAINER_CYLINDER_Z_ABSORB;
                 const double ex = x[i] - cx, ey = y[i] - cy, ez = z[i] - cz;
                 double sx = 0.0, sy = 0.0, sz = 0.0;
                 const double rr = ex * ex + ey * ey;
@@ -295,11 +305,28 @@ int ora_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
                         const double q = 1.0 / sqrt(rr);
                         sx = ex * (R0 * q - 1.0);
                         sy = ey * (R0 * q - 1.0);
-                        s.vx[i] *= -1.0;
-                        s.vy[i] *= -1.0;
+                        if (absorb) {
+                            const double ux = ex * q, uy = ey * q;
+                            const double vr = s.vx[i] * ux + s.vy[i] * uy;
+                            if (vr > 0.0) {
+                                s.vx[i] -= vr * ux;
+                                s.vy[i] -= vr * uy;
+                            }
+                        } else {
+                            s.vx[i] *= -1.0;
+                            s.vy[i] *= -1.0;
+                        }
                     }
-                    if (ez > H0) { sz = H0 - ez; s.vz[i] *= -1.0; }
-                    if (ez < -H0) { sz = -H0 - ez; s.vz[i] *= -1.0; }
+                    if (ez > H0) {
+                        sz = H0 - ez;
+                        if (!absorb) s.vz[i] *= -1.0;
+                        else if (s.vz[i] > 0.0) s.vz[i] = 0.0;
+                    }
+                    if (ez < -H0) {
+                        sz = -H0 - ez;
+                        if (!absorb) s.vz[i] *= -1.0;
+                        else if (s.vz[i] < 0.0) s.vz[i] = 0.0;
+                    }
                     x[i] += sx; y[i] += sy; z[i] += sz;
                 }
             }

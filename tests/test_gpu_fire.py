@@ -31,7 +31,11 @@ def packing(n, seed, R0=4.0, H0=4.0):
 
 
 def run_both(xyz, r, **kw):
-    g = fire.defaults(D=1.0, density=2.65, **kw)
+    return run_both_x(xyz, r, **kw)[0]
+
+
+def run_both_x(xyz, r, **kw):
+    g = fire.defaults(**dict(dict(D=1.0, density=2.65), **kw))
     o = pyoracle.fire_defaults()
     for k, v in abi.struct_to_dict(g).items():
         if k in ("center", "hidden"):
@@ -45,7 +49,7 @@ def run_both(xyz, r, **kw):
         assert getattr(gout, f) == getattr(oout, f), (f, getattr(gout, f), getattr(oout, f))
     assert np.array_equal(gx, ox), float(np.abs(gx - ox).max())
     assert np.array_equal(gv, ov), float(np.abs(gv - ov).max())
-    return gout
+    return gout, gx
 
 
 @pytest.fixture(autouse=True)
@@ -78,3 +82,22 @@ def test_fire_removal():
 def test_fire_tiny_skin_rebuilds():
     xyz, r = packing(2000, 4)
     run_both(xyz, r, dt0=0.02, quirks=0, max_steps=300, check_every=7, skin=0.002)
+
+
+def test_fire_harmonic_absorbing_growth():
+    """The command line's packing growth (harmonic law, absorbing walls) at
+    two radii and the final stage with removal: bitwise equal at each stage."""
+    rng = np.random.default_rng(12)
+    R0, H0, n = 6.0, 8.0, 2500  # packing fraction 0.58 at the final radius
+    ang = rng.uniform(0, 2 * np.pi, n)
+    rad = R0 * np.sqrt(rng.uniform(0, 1, n))
+    xyz = np.stack([rad * np.cos(ang), rad * np.sin(ang), rng.uniform(-H0, H0, n)], 1)
+    for sc, last in ((0.6, False), (0.85, False), (1.0, True)):
+        out, xyz = run_both_x(xyz, np.full(n, 0.5 * sc), dt0=0.02, quirks=0, density=2.61,
+                              law=abi.RF_FIRE_LAW_HARMONIC,
+                              container=abi.RF_CONTAINER_CYLINDER_Z_ABSORB, radius=R0,
+                              half_height=H0, hide_z=1e9, max_steps=50,
+                              hard_cap=1500 if last else 3000,
+                              gap_tol=-1e-4 if last else -1e-3, check_every=50 if last else 20,
+                              delete_after=500 if last else -1, hidden=(500.0, 500.0, 500.0))
+        assert out.steps > 0

@@ -154,3 +154,41 @@ def test_graph_replay_parity():
     kms, timed, _, _ = ctxs[2].last_timing()
     assert timed > 0 and kms > 0.0
     assert ctxs[0].counters().rebuilds > 1
+
+
+def test_ejected_particles_parity():
+    """Sixty surface spheres shot outward leave the space box and are parked
+    at the hidden point (d_ParallelepipedCutRestriction): more hidden
+    particles than a candidate list holds, all at one point.  State, curve
+    and controller stay bitwise equal, and parked particles do not keep
+    asking for list rebuilds."""
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="none", seed=8)
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    free = abi.PlatenCfg()
+    free.kind = abi.RF_TEST_NONE
+    g.set_test(free)
+    o.set_test(free)
+    st = g.get_state()
+    n = g.n
+    R = st["R"].reshape(3, n)
+    c = R.mean(axis=1)
+    rad = np.hypot(R[0] - c[0], R[1] - c[1])
+    out = np.argsort(-rad, kind="stable")[:60]
+    V = st["V"].reshape(3, n)
+    V[0, out] = 0.8 * (R[0, out] - c[0]) / rad[out]
+    V[1, out] = 0.8 * (R[1, out] - c[1]) / rad[out]
+    g.set_state(st)
+    o.set_state(st)
+    hid = np.array(w["box"].hidden[:])
+    for chunk in (50, 250, 700):
+        assert g.step(chunk) == o.step(chunk) == chunk
+        compare_states(g.get_state(), o.get_state(), n, g.slots, what=f"ejected +{chunk}")
+    R = g.get_state()["R"].reshape(3, n)
+    parked = np.all(R.T == hid, axis=1).sum()
+    assert parked > 32, parked
+    before = g.counters().rebuilds
+    assert g.step(500) == o.step(500) == 500
+    compare_states(g.get_state(), o.get_state(), n, g.slots, what="ejected, parked")
+    assert g.counters().rebuilds - before < 50

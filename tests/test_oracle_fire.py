@@ -150,3 +150,114 @@ def test_removal_of_the_worst_pair():
     assert np.array_equal(x1[1], [100.0, 100.0, 100.0]) and np.all(v1[1] == 0.0)
     assert np.array_equal(x1[2], [5.0, 0.0, 0.0])
     assert out.min_gap == 0.0 and out.stop_reason == 1
+
+
+def test_harmonic_law_first_step():
+    """Harmonic law: force D (a - rho) along the normal.  One inertial step
+    from rest moves each sphere by F dt0^2 / m (alpha mixing of a velocity
+    parallel to F is the identity), against the reference law's D a^2/rho."""
+    d, r, D, dt0 = 0.4, 0.25, 3.0, 1e-3
+    xyz, rr = pair(d, r)
+    c = cfg(max_steps=1, quirks=0, D=D, dt0=dt0, law=abi.RF_FIRE_LAW_HARMONIC)
+    x1, v1, out = pyoracle.fire_relax(xyz, rr, c)
+    F0 = -D * (2 * r - d)
+    v = F0 * dt0 / mass(r)
+    assert v1[0, 0] == pytest.approx(v, rel=1e-13)
+    assert x1[0, 0] == pytest.approx(v * dt0, rel=1e-12)
+    assert x1[1, 0] == pytest.approx(d - v * dt0, rel=1e-12)
+    x2, _, _ = pyoracle.fire_relax(xyz, rr, cfg(max_steps=1, quirks=0, D=D, dt0=dt0))
+    ratio = x2[0, 0] / x1[0, 0]
+    assert ratio == pytest.approx((2 * r) ** 2 / d / (2 * r - d), rel=1e-12)
+
+
+def test_harmonic_force_vanishes_at_contact():
+    """A pair exactly at contact distance feels no force under either law
+    (rho^2 < a^2 fails), and a pair just inside contact under the harmonic
+    law feels a force proportional to the overlap."""
+    for law in (abi.RF_FIRE_LAW_REFERENCE, abi.RF_FIRE_LAW_HARMONIC):
+        xyz, rr = pair(0.5)
+        x1, v1, _ = pyoracle.fire_relax(xyz, rr, cfg(max_steps=1, quirks=0, law=law))
+        assert np.array_equal(x1, xyz) and np.all(v1 == 0.0)
+    dx = []
+    for ov in (1e-3, 2e-3):
+        xyz, rr = pair(0.5 - ov)
+        x1, _, _ = pyoracle.fire_relax(xyz, rr, cfg(max_steps=1, quirks=0,
+                                                     law=abi.RF_FIRE_LAW_HARMONIC))
+        dx.append(-x1[0, 0])
+    assert dx[1] / dx[0] == pytest.approx(2.0, rel=1e-9)
+
+
+def test_absorbing_wall_removes_only_the_outward_velocity():
+    """A sphere pushed through the cylinder wall by its neighbour is put back
+    on the wall; with the absorbing container its velocity keeps the
+    tangential part and loses the outward normal part, with the reflecting
+    container (d_CylinderRestrictionZ) the whole radial-plane velocity is
+    reversed."""
+    # neighbour at the axis side pushes sphere 1 outward along +x, with a
+    # y component that is tangential at the wall point
+    xyz = np.array([[1.62, -0.3, 0.0], [1.9, 0.0, 0.0]])
+    rr = np.array([0.25, 0.25])
+    base = dict(max_steps=1, quirks=0, D=5.0, dt0=0.2, radius=1.9, half_height=5.0, hide_z=50.0)
+    xa, va, _ = pyoracle.fire_relax(xyz, rr, cfg(container=abi.RF_CONTAINER_CYLINDER_Z_ABSORB,
+                                                 **base))
+    xr, vr, _ = pyoracle.fire_relax(xyz, rr, cfg(container=abi.RF_CONTAINER_CYLINDER_Z, **base))
+    xn, vn, _ = pyoracle.fire_relax(xyz, rr, cfg(container=abi.RF_CONTAINER_NONE, **base))
+    # free motion leaves the cylinder
+    assert np.hypot(xn[1, 0], xn[1, 1]) > 1.9
+    # both containers put the centre back on the wall at the same point
+    assert np.hypot(xa[1, 0], xa[1, 1]) == pytest.approx(1.9, rel=1e-15)
+    np.testing.assert_array_equal(xa[1], xr[1])
+    u = xa[1, :2] / np.hypot(xa[1, 0], xa[1, 1])
+    v = vn[1, :2]
+    vrad = v @ u
+    assert vrad > 0.0
+    np.testing.assert_allclose(va[1, :2], v - vrad * u, rtol=1e-13, atol=1e-16)
+    assert abs(va[1, :2] @ u) < 1e-14 * np.abs(v).max()
+    np.testing.assert_array_equal(vr[1, :2], -v)
+    # the end planes: outward z velocity is removed, inward kept
+    xyz = np.array([[0.0, 0.0, 4.72], [0.0, 0.0, 4.98]])
+    base.update(radius=5.0, half_height=4.98)
+    xa, va, _ = pyoracle.fire_relax(xyz, rr, cfg(container=abi.RF_CONTAINER_CYLINDER_Z_ABSORB,
+                                                 **base))
+    xn, vn, _ = pyoracle.fire_relax(xyz, rr, cfg(container=abi.RF_CONTAINER_NONE, **base))
+    assert xn[1, 2] > 4.98 and xa[1, 2] == 4.98
+    assert va[1, 2] == 0.0 and vn[1, 2] > 0.0
+    assert va[0, 2] == vn[0, 2] < 0.0
+
+
+def test_dense_growth_converges_with_harmonic_law():
+    """Radius growth of a random packing in a cylinder (the command line's
+    sample creation, 0.6 -> 0.92 of the final radius): with the harmonic law
+    and absorbing walls every stage converges to overlaps below 0.1 %
+    (checked by brute force); the reference law with reflecting walls stalls
+    at the minimum step."""
+    rng = np.random.default_rng(11)
+    R0, H0, rf = 3.0, 4.0, 0.5
+    n = int(0.6 * np.pi * (R0 + rf) ** 2 * (2 * H0 + 2 * rf) / (4.0 / 3.0 * np.pi * rf ** 3))
+    pts = []
+    while len(pts) < n:
+        X, Y = rng.uniform(-R0, R0, 2)
+        if X * X + Y * Y <= R0 * R0:
+            pts.append((X, Y, rng.uniform(-H0, H0)))
+    start = np.array(pts)
+
+    def grow(law, container):
+        xyz = start.copy()
+        for sc in (0.6, 0.75, 0.85, 0.92):
+            c = cfg(quirks=0, D=1.0, dt0=0.02, density=2.61, law=law, container=container,
+                    radius=R0, half_height=H0, hide_z=1e30, max_steps=50, hard_cap=3000,
+                    gap_tol=-1e-3, check_every=20)
+            xyz, _, out = pyoracle.fire_relax(xyz, np.full(n, sc * rf), c, threads=4)
+            if out.stop_reason != 1:
+                return xyz, out, sc
+        return xyz, out, None
+
+    xyz, out, failed = grow(abi.RF_FIRE_LAW_HARMONIC, abi.RF_CONTAINER_CYLINDER_Z_ABSORB)
+    assert failed is None
+    a = 2 * 0.92 * rf
+    d = np.linalg.norm(xyz[:, None] - xyz[None], axis=2) + np.eye(n) * 1e9
+    assert ((d - a) / a).min() >= -1e-3
+    assert np.all(np.hypot(xyz[:, 0], xyz[:, 1]) <= R0 * (1 + 1e-15))
+    assert np.all(np.abs(xyz[:, 2]) <= H0)
+    _, out_ref, failed_ref = grow(abi.RF_FIRE_LAW_REFERENCE, abi.RF_CONTAINER_CYLINDER_Z)
+    assert failed_ref is not None and out_ref.dt == pytest.approx(0.02 * 1e-3)

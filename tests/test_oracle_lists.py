@@ -31,6 +31,12 @@ def _xyz(st, n):
     return st["R"].reshape(3, n).T.copy()
 
 
+def working_grid(b):
+    """The reference grid (dims) plus one cell layer past the top x and y
+    faces, which the oracle and the kernels index with."""
+    return b.dims[0] + 1, b.dims[1] + 1
+
+
 @pytest.fixture(scope="module")
 def wl():
     return config.make_workload(n=600, r_min=0.2, r_max=0.3, test="none",
@@ -47,7 +53,9 @@ def test_cell_sort_is_stable_by_key(wl):
     b = wl["box"]
     a = b.cell
     ix = np.floor(wl["xyz"] / a).astype(int)
-    assert np.array_equal(key, ix[:, 0] + ix[:, 1] * b.dims[0] + ix[:, 2] * b.dims[0] * b.dims[1])
+    nx, ny = working_grid(b)
+    assert np.array_equal(key, ix[:, 0] + ix[:, 1] * nx + ix[:, 2] * nx * ny)
+    assert len(ct) == nx * ny * b.dims[2]
 
 
 def test_construct_rows_equal_bruteforce(wl):
@@ -70,7 +78,8 @@ def test_construct_row_order_follows_stencil(wl):
     key, _, _, _ = o.cells()
     b = wl["box"]
     a = b.cell
-    nx, nxy = b.dims[0], b.dims[0] * b.dims[1]
+    nx, ny = working_grid(b)
+    nxy = nx * ny
     xyz = wl["xyz"]
     for i in range(0, o.n, 37):
         f = np.floor(xyz[i] / a)
