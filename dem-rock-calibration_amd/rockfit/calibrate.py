@@ -1,0 +1,136 @@
+"""Calibration on top of the GPU loading step.
+
+Two drivers sit above the C-ABI:
+
+- the reference's own: the conjugate-gradient step over finite differences
+  (DetermineConjGradient, mp_determineconjgradient.cu:146-368, mp_task.h:233-311)
+  is `RockFit-DEM DCG` (cli/rf_cg.h), whose difference runs go to the GPU
+  as one batch; the surrogate farm (NN_MP.py) drives `RockFit-DEM M(...)`
+  (rockfit.farm) and scores results with `target_functional` below;
+- a gradient one: `fit_curve` matches a loading curve with Gauss-Newton /
+  Levenberg-Marquardt steps whose Jacobian is the forward-mode tangent of the
+  same GPU run (rf_sens_*), one run per iteration instead of NParam + 1.
+"""
+import numpy as np
+
+from . import autograd
+
+# ResultCalculateOneSample fields compared by the farm's objective
+# (NN_MP.py:849): strain_u, stress_u, linearity_u, destruction_u, stress_b,
+# destruction_b.
+FARM_FEATURES = (6, 7, 8, 9, 11, 13)
+
+
+def farm_features(row):
+    """The six compared quantities of one ResultCalculateOneSample row (14
+    values)."""
+    row = np.asarray(row, dtype=np.float64)
+    return row[list(FARM_FEATURES)]
+
+
+def target_functional(points, target, weights):
+    """ReCalculateTargetFunctional (NN_MP.py:461-504): per point the sum over
+    six quantities of weight * d^2 with
+      d = x/t - 1 if x >= t else t/x - 1       (quantities 0, 1, 2, 4),
+      d = 10 if 1 - x < t or 1 - x > 0.7 else 0 (quantity 3, destruction),
+      d = 10 if 1 - x < t else 0               (quantity 5),
+    each d capped at 10."""
+    out = []
+    for x in points:
+        y = 0.0
+        for j in range(6):
+            if j == 3:
+                d = 10.0 if (1.0 - x[j] < target[j] or 1.0 - x[j] > 0.7) else 0.0
+            elif j == 5:
+                d = 10.0 if 1.0 - x[j] < target[j] else 0.0
+            elif x[j] >= target[j]:
+                d = x[j] / target[j] - 1.0
+            else:
+                d = target[j] / x[j] - 1.0
+            if d > 10:
+                d = 10.0
+            y += weights[j] * d * d
+        out.append(y)
+    return out
+
+
+def normalized_parameters(p, coeffs, n):
+    """NormalizedParameters (NN_MP.py:408-421)."""
+    return [p[i] / coeffs[i] for i in range(n)]
+
+
+def renormalized_parameters(p, coeffs, n):
+    """ReNormalizedParameters (NN_MP.py:424-429)."""
+    return [p[i] * coeffs[i] for i in range(n)]
+
+
+def local_points(X, Y, variation, external_variation, n_min, n_param):
+    """GetNewArrayOfPoints (NN_MP.py:506-543): the best point, the points
+    inside its external window [e/v_ext, e*v_ext] on every parameter, and
+    how many random runs are missing to reach n_min.  Returns
+    (n_random, X_local, Y_local, border, index_best)."""
+    best = int(np.argmin(Y))
+    m = X[best]
+    border = [(e / variation, e * variation) for e in m]
+    outer = [(e / external_variation, e * external_variation) for e in m]
+    Xl, Yl = [], []
+    for i in range(len(X)):
+        e = X[i]
+        if all(outer[j][0] <= e[j] <= outer[j][1] for j in range(n_param)):
+            Xl.append(X[i])
+            Yl.append(Y[i])
+    n_rand = max(n_min - len(Yl), 0)
+    return n_rand, Xl, Yl, border, best
+
+
+def cg_functional(results, goal, weight_mult):
+    """The CG's weighted squares W_j (G_j - R_j)^2 with W_j = mult_j / G_j^2
+    (CalculateStartParameters and CalculateNewParameters2, mp_task.h:138-139,
+    275-279)."""
+    goal = np.asarray(goal, dtype=np.float64)
+    W = np.asarray(weight_mult, dtype=np.float64) / goal ** 2
+    D = goal - np.asarray(results, dtype=np.float64)
+    return W * D * D
+
+
+def fit_curve(setup, target, names, theta0, iters=10, lam=1e-3, tol=1e-10, scale=None,
+              verbose=False):
+    """Fit material parameters to a loading curve on the GPU.
+
+    setup: dict(xyz, r, mat (dt fixed), box, cfg, steps) as for
+    autograd.run_window; target: (points, 2) platen stresses in reduced
+    units; names: parameters (autograd.PARAMS) started at theta0.
+    Levenberg-Marquardt in the scaled variables theta / scale: each
+    iteration is one GPU run that returns the curve and its tangents.
+    Returns (theta, history) with history = [(loss, theta), ...]."""
+    theta = np.asarray(theta0, dtype=np.float64).copy()
+    scale = np.abs(theta) if scale is None else np.asarray(scale, dtype=np.float64)
+    target = np.asarray(target, dtype=np.float64)
+    stress, J = autograd.run_window(setup, names, theta, sens=True)
+    r = (stress - target).ravel()
+    loss = float(r @ r)
+    history = [(loss, theta.copy())]
+    for it in range(iters):
+        # J: (points, nparam, 2) -> residual rows (points*2, nparam), scaled
+        A = np.transpose(J, (0, 2, 1)).reshape(-1, len(names)) * scale
+        H = A.T @ A
+        g = A.T @ r
+        while True:
+            step = np.linalg.solve(H + lam * np.diag(np.diag(H)), -g)
+            trial = theta + step * scale
+            s2, J2 = autograd.run_window(setup, names, trial, sens=True)
+            r2 = (s2 - target).ravel()
+            loss2 = float(r2 @ r2)
+            if verbose:
+                print(f"iter {it} lam {lam:.1e} loss {loss:.6e} -> {loss2:.6e} theta {trial}")
+            if loss2 < loss:
+                theta, stress, J, r, loss = trial, s2, J2, r2, loss2
+                lam = max(lam / 10.0, 1e-12)
+                break
+            lam *= 10.0
+            if lam > 1e8:
+                return theta, history
+        history.append((loss, theta.copy()))
+        if loss <= tol * history[0][0]:
+            break
+    return theta, history

@@ -8,6 +8,7 @@ placed at the centre of the reference's space boxes (40x40x60 mm uniaxial,
 65x65x30 mm Brazilian; mp_calculateonesample.cu:91).
 """
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -111,3 +112,97 @@ def test_cli_brazilian_equals_api(binary, tmp_path):
             ts.peak_stress * 1e9, 0, ts.destruction]
     assert res["result_text"] == ",".join(fmt(v) for v in want)
     check_curve_file(str(tmp_path / "result" / "Brazil_0.txt"), curve)
+
+
+def cg_box(kind):
+    """DetermineConjGradient's run box: space box set up with cell 2 x 3.2 p_r,
+    re-gridded with 2 x 2.9 p_r."""
+    b = core.box_reference(SPACE[kind], 2.0 * 3.2 * P_R)
+    cell = 2.0 * 2.9 * P_R
+    b.cell = cell
+    for d in range(3):
+        B = b.center[d] + b.half[d]
+        b.dims[d] = int(np.floor(B * (1.0 / cell))) + (4 if d == 2 else 1)
+    return b
+
+
+def cg_material(P):
+    m = config.reference_material()
+    m.E_bond = m.E_contact = P[0]
+    m.sigma_c, m.tau_c, m.bond_radius, m.bond_gap = P[1], P[2], P[3], P[4]
+    m.acut = 2.9 * P_R
+    return m
+
+
+def test_cli_dcg_step_uses_the_right_runs(binary, tmp_path):
+    """Two steps of DetermineConjGradient on a small uniaxial sample: the
+    base run and the three varied runs (type 7: sigma_c, tau_c, b_r, each
+    +10 %) go to the GPU as one batch; the recorded results and functional
+    equal single runs through the API, and the next point is the CG update
+    of those results (goals and weights of mp_determineconjgradient.cu:150-160,
+    273-276)."""
+    print("case cli DCG n=1500 seed=9 steps=2", flush=True)
+    os.makedirs(tmp_path / "result")
+    path = str(tmp_path / "result" / "Sample_0.txt")
+    xyz0, _ = sample.generate(1500, P_R, P_R, shape="cylinder", aspect=2.0, seed=9, relax="gpu")
+    cbox = core.box_reference(SPACE[0], 2.0 * 3.2 * P_R)
+    sample.write_sample(path, xyz0 + np.array(cbox.center[:]))
+    env = dict(os.environ, RF_CG_STEPS="2", RF_CG_REUSE_SAMPLES="1")
+    p = subprocess.run([binary, "G0", "DCG"], cwd=str(tmp_path), env=env, capture_output=True,
+                       text=True, timeout=600)
+    assert p.returncode == 0, p.stderr[-3000:]
+    lines = open(tmp_path / "result" / "CalculationResults.dat").read().splitlines()
+    assert len(lines) == 8
+    xyz, _ = sample.read_sample(path)
+    r = np.full(len(xyz), P_R)
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    cfg = config.uniaxial_cfg(xyz, r, v_load_mps=-0.1, min_stress_mpa=10.0, max_strain=0.025,
+                              cond_fracture=0.2, dnsave=100)
+    for k in range(3):
+        cfg.size[k] = hi[k] - lo[k] + 2.0 * P_R
+        cfg.center[k] = 0.5 * (hi[k] + lo[k])
+    box = cg_box(0)
+
+    def results(P):
+        g = Context(xyz, r, cg_material(P), box)
+        g.build_bonds()
+        g.set_test(cfg)
+        ts = g.run()
+        g.close()
+        return np.array([ts.peak_strain, ts.peak_stress, 1.0, ts.linearity_deviation])
+
+    goal = np.array([0.00480002, 104.0497482e6 / 1e9, 10.5e6 / 1e9, 1.0])
+    W = np.array([50.0, 100.0, 1.0, 10000.0]) / goal ** 2
+    W[2], goal[2] = 100.0, 1.0
+    varied = np.array([0, 1, 1, 1, 0])
+    P = np.array([0.45 * 1.71066e10 / 1e9, 1.0 * 6.18389e8 / 1e9, 1.0 * 5.03332e8 / 1e9,
+                  4 * 0.000409345 / 1e-3, 0.00046797 / 1e-3])
+    dF_old = np.zeros(5)
+    for step in range(2):
+        base = lines[4 * step:4 * step + 4]
+        assert base[0] == f"Step: {step}"
+        got = np.array([float(v) for v in base[1].split(":")[1].split()])
+        np.testing.assert_allclose(got, P * np.array([1e9, 1e9, 1e9, 1e-3, 1e-3]), rtol=1e-5)
+        var = varied * 0.1 * P
+        R0 = results(P)
+        assert R0[1] > 0.0
+        want = "%g %g %g" % (R0[0], R0[1] * 1e9, R0[2] * 1e9)
+        assert base[2].split(": ")[1] == want
+        dR = np.zeros((5, 4))
+        for i in range(5):
+            if var[i] > 1e-12:
+                Pi = P.copy()
+                Pi[i] += var[i]
+                dR[i] = (results(Pi) - R0) / var[i]
+        D = goal - R0
+        F = W * D * D
+        got_F = np.array([float(v) for v in base[3].split()[1:5]])
+        np.testing.assert_allclose(got_F, F, rtol=1e-5, atol=1e-12)
+        g = -2.0 * dR @ (W * D)
+        beta = (g @ g) / (dF_old @ dF_old) if dF_old @ dF_old > 1e-12 else 0.0
+        dd = g + beta * dF_old
+        s = dR.T @ dd
+        t = -np.sum(W * D * s) / np.sum(W * s * s)
+        P = P - t * dd
+        P = np.maximum(P, [1e5 / 1e9, 1e5 / 1e9, 1e8 / 1e9, 1e-6 / 1e-3, 1e-5 / 1e-3])
+        dF_old = g
