@@ -719,6 +719,26 @@ rf_status rf_last_timing(rf_ctx* c, double* kms, int64_t* timed, double* tms, in
 
 namespace {
 
+// CalculateLinearityDeviation (md_results.cu:147-165) over n stored samples
+// (later ones read as zero); the loop also stops at `count` samples
+// (stepsave).
+double linearity_deviation(const double* h, int64_t n, int64_t count, double peak_strain,
+                           double peak_stress) {
+    auto at = [&](int64_t i, int c) { return i < n ? h[3 * i + c] : 0.0; };
+    double l_eps = 0, l_deps = 0, l_A = peak_stress / peak_strain, l_dS = 0, ld = 0;
+    int64_t l_i = 0;
+    do {
+        ld += l_deps * l_dS * l_dS;
+        ++l_i;
+        l_eps = -at(l_i, 0);
+        l_deps = -0.5 * (at(l_i + 1, 0) - at(l_i - 1, 0));
+        l_dS = (l_A * l_eps - 0.5 * (at(l_i, 2) - at(l_i, 1)));
+    } while (l_eps < peak_strain - 2 * l_deps && l_i < count);
+    ld /= (peak_strain * peak_stress * peak_stress);
+    ld += 1.0;
+    return ld;
+}
+
 // CalculateLinearityDeviation and CalculateDestruction (md_results.cu:147-187)
 // on the downloaded curve; zero-filled past the last sample.
 rf_status post_process(rf_ctx* c) {
@@ -729,21 +749,9 @@ rf_status post_process(rf_ctx* c) {
     HIP_TRY(hipMemcpy(h.data(), c->d.curve, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bc.data(), c->d.bcount, bc.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     const rf::Ctrl& t = c->hctrl;
-    if (c->cfg.kind == RF_TEST_UNIAXIAL) {
-        double l_eps = 0, l_deps = 0, l_A = t.peak_stress / t.peak_strain, l_dS = 0, ld = 0;
-        int64_t l_i = 0;
-        do {
-            ld += l_deps * l_dS * l_dS;
-            ++l_i;
-            if (l_i + 1 > cap + 1) break;
-            l_eps = -h[3 * l_i];
-            l_deps = -0.5 * (h[3 * (l_i + 1)] - h[3 * (l_i - 1)]);
-            l_dS = (l_A * l_eps - 0.5 * (h[3 * l_i + 2] - h[3 * l_i + 1]));
-        } while (l_eps < t.peak_strain - 2 * l_deps && l_i < t.stepsave);
-        ld /= (t.peak_strain * t.peak_stress * t.peak_stress);
-        ld += 1.0;
-        c->ld = ld;
-    }
+    if (c->cfg.kind == RF_TEST_UNIAXIAL)
+        c->ld = linearity_deviation(h.data(), std::min<int64_t>(t.stepsave, cap), t.stepsave,
+                                    t.peak_strain, t.peak_stress);
     if (t.stepsave >= 1 && bc[0] > 0) c->destruction = (double)bc[t.stepsave - 1] / (double)bc[0];
     return RF_OK;
 }
@@ -764,6 +772,47 @@ void fill_state(const rf_ctx* c, rf_test_state* o) {
 }  // namespace
 
 extern "C" {
+
+rf_status rf_linearity_deviation(const double* curve, int64_t n, double peak_strain,
+                                 double peak_stress, double* out) {
+    if (!out || n < 0 || (n > 0 && !curve)) return fail(RF_ERR_ARG, "rf_linearity_deviation: bad arguments");
+    if (!(peak_strain != 0.0) || !(peak_stress != 0.0))
+        return fail(RF_ERR_ARG, "rf_linearity_deviation: zero peak strain or stress");
+    *out = linearity_deviation(curve, n, n, peak_strain, peak_stress);
+    return RF_OK;
+}
+
+rf_status rf_curve_moduli(const double* curve, int64_t n, double peak_strain, double peak_stress,
+                          double lo, double hi, rf_moduli* out) {
+    if (!out || n < 0 || (n > 0 && !curve) || !(lo <= hi))
+        return fail(RF_ERR_ARG, "rf_curve_moduli: bad arguments");
+    *out = rf_moduli{};
+    if (!(peak_strain != 0.0)) return fail(RF_ERR_ARG, "rf_curve_moduli: zero peak strain");
+    out->secant = peak_stress / peak_strain;
+    // ascending branch: up to the first sample at the peak stress
+    int64_t end = n;
+    for (int64_t i = 0; i < n; ++i)
+        if (0.5 * (curve[3 * i + 2] - curve[3 * i + 1]) >= peak_stress) {
+            end = i + 1;
+            break;
+        }
+    double sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
+    int64_t m = 0;
+    for (int64_t i = 0; i < end; ++i) {
+        const double e = -curve[3 * i], s = 0.5 * (curve[3 * i + 2] - curve[3 * i + 1]);
+        if (s < lo * peak_stress || s > hi * peak_stress) continue;
+        sx += e; sy += s; sxx += e * e; sxy += e * s; syy += s * s;
+        ++m;
+    }
+    out->points = m;
+    if (m < 2) return RF_OK;
+    const double cxx = sxx - sx * sx / m, cxy = sxy - sx * sy / m, cyy = syy - sy * sy / m;
+    if (!(cxx > 0.0)) return RF_OK;
+    out->lsq = cxy / cxx;
+    out->lsq_intercept = (sy - out->lsq * sx) / m;
+    out->r2 = cyy > 0.0 ? cxy * cxy / (cxx * cyy) : 1.0;
+    return RF_OK;
+}
 
 rf_status rf_run(rf_ctx* c, rf_test_state* out) {
     if (!c) return fail(RF_ERR_ARG, "null context");

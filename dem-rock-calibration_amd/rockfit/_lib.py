@@ -29,6 +29,7 @@ EXPORTS = [
     "rf_fire_defaults", "rf_fire_relax", "rf_set_graph",
     "rf_sens_material_tangent", "rf_sens_begin", "rf_sens_step", "rf_sens_get_curve",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
+    "rf_linearity_deviation", "rf_curve_moduli",
 ]
 
 
@@ -51,6 +52,12 @@ def lib():
     L.rf_derive.restype = st
     L.rf_box_reference.argtypes = [P(C.c_double), C.c_double, P(abi.Box)]
     L.rf_box_reference.restype = st
+    L.rf_linearity_deviation.argtypes = [P(C.c_double), C.c_int64, C.c_double, C.c_double,
+                                         P(C.c_double)]
+    L.rf_linearity_deviation.restype = st
+    L.rf_curve_moduli.argtypes = [P(C.c_double), C.c_int64, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, P(abi.Moduli)]
+    L.rf_curve_moduli.restype = st
     L.rf_create.argtypes = [P(abi.Sample), P(abi.Material), P(abi.Box), P(abi.Options), P(vp)]
     L.rf_create.restype = st
     L.rf_destroy.argtypes = [vp]

@@ -140,6 +140,11 @@ class FireCfg(C.Structure):
     ]
 
 
+class Moduli(C.Structure):
+    _fields_ = [("secant", C.c_double), ("lsq", C.c_double), ("lsq_intercept", C.c_double),
+                ("r2", C.c_double), ("points", C.c_int64)]
+
+
 class FireResult(C.Structure):
     _fields_ = [
         ("steps", C.c_int64), ("np_positive", C.c_int64), ("np_negative", C.c_int64),

@@ -53,6 +53,30 @@ def box_reference(space, cell):
     return b
 
 
+def _curve_ptr(curve):
+    c = np.ascontiguousarray(np.asarray(curve, dtype=np.float64).reshape(-1, 3))
+    return c, c.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def linearity_deviation(curve, peak_strain, peak_stress):
+    """CalculateLinearityDeviation of a curve (n, 3) in the rf_get_curve layout."""
+    c, ptr = _curve_ptr(curve)
+    out = C.c_double()
+    check(lib().rf_linearity_deviation(ptr, len(c), float(peak_strain), float(peak_stress),
+                                       C.byref(out)), "rf_linearity_deviation")
+    return out.value
+
+
+def curve_moduli(curve, peak_strain, peak_stress, lo=0.1, hi=0.5):
+    """Secant modulus and least-squares modulus over [lo, hi] x peak stress of
+    the ascending branch (rf_curve_moduli)."""
+    c, ptr = _curve_ptr(curve)
+    out = abi.Moduli()
+    check(lib().rf_curve_moduli(ptr, len(c), float(peak_strain), float(peak_stress), float(lo),
+                                float(hi), C.byref(out)), "rf_curve_moduli")
+    return out
+
+
 class Context:
     """One sample on the GPU."""
 

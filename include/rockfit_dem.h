@@ -221,6 +221,33 @@ rf_status rf_get_curve(rf_ctx* ctx, double* out, int64_t cap, int64_t* n);
 rf_status rf_get_bond_counts(rf_ctx* ctx, int64_t* out, int64_t cap, int64_t* n);
 
 rf_status rf_get_counters(rf_ctx* ctx, rf_counters* out);
+
+/* Curve post-processing on the host (no device needed).  `curve` holds n
+ * samples of 3 doubles (strain, bottom stress, top stress) as rf_get_curve
+ * returns them; samples past n count as zero, as in the reference's
+ * zero-filled buffer.
+ *
+ * CalculateLinearityDeviation (md_results.cu:147-165): with the secant
+ * A = peak_stress / peak_strain, 1 + sum over samples of
+ * deps_i (A eps_i - s_i)^2 / (peak_strain peak_stress^2), eps_i = -strain,
+ * s_i = (top - bottom)/2, deps_i the central difference of eps; the sum runs
+ * while eps_i < peak_strain - 2 deps_i.  rf_run reports the same value for
+ * uniaxial tests. */
+rf_status rf_linearity_deviation(const double* curve, int64_t n, double peak_strain,
+                                 double peak_stress, double* out);
+
+/* Moduli of a loading curve: the reference's secant peak_stress/peak_strain
+ * (md_results.cu:149) and, without a reference counterpart, the least-squares
+ * slope (and intercept) of s_i against eps_i over the ascending branch (the
+ * samples up to the first one at peak_stress) whose stress lies in
+ * [lo, hi] x peak_stress, with its coefficient of determination. */
+typedef struct rf_moduli {
+    double secant;
+    double lsq, lsq_intercept, r2;
+    int64_t points;      /* samples in the fit */
+} rf_moduli;
+rf_status rf_curve_moduli(const double* curve, int64_t n, double peak_strain, double peak_stress,
+                          double lo, double hi, rf_moduli* out);
 rf_status rf_get_state(rf_ctx* ctx, rf_state_view* out);
 rf_status rf_set_state(rf_ctx* ctx, const rf_state_view* in);
 int64_t rf_num_particles(const rf_ctx* ctx);
