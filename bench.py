@@ -169,6 +169,92 @@ def batch_bench(count, steps, warmup):
     }))
 
 
+def grad_bench(steps=2000, n=20000):
+    """Cost of the loading-response gradient against the forward run it
+    differentiates: a 20k-sphere cylinder (r ~ U[0.20, 0.30] mm) loaded for
+    2000 steps to 0.1 % strain (closing speed 0.001 H / (2000 dt)), three
+    parameter directions (bond E, bond radius ratio, contact E).  Both runs
+    step the same window from the same built state; the gradient run carries
+    the tangents (rf_sens_*).  Also compares the tangent of the axial stress
+    at each curve point with central differences (relative step 1e-2) over
+    the points before the controller first pauses a platen."""
+    import numpy as np
+
+    from rockfit import autograd, config
+    from rockfit.core import Context
+
+    w = config.make_workload(n=n, r_min=0.20, r_max=0.30, test="uniaxial", seed=5,
+                             gen_kwargs=dict(relax="gpu"),
+                             test_kwargs=dict(dnsave=50, max_steps=100000, max_strain=0.05,
+                                              min_stress_mpa=0.1))
+    mat = autograd.fixed_dt(w["mat"], w["r"])
+    cfg = w["cfg"]
+    cfg.v_load = -0.001 * cfg.size[2] / (steps * mat.dt)
+    names = ["E_bond", "bond_radius_ratio", "E_contact"]
+    theta = np.array([getattr(mat, autograd.PARAMS[k][1]) for k in names])
+
+    def timed(sens):
+        c = Context(w["xyz"], w["r"], mat, w["box"])
+        c.build_bonds()
+        c.set_test(cfg)
+        if sens:
+            c.sens_begin([autograd.PARAMS[k][0] for k in names])
+        device_sync()
+        t = time.perf_counter()
+        done = c.sens_step(steps) if sens else c.step(steps)
+        device_sync()
+        dt = time.perf_counter() - t
+        c.close()
+        return dt, done
+
+    timed(False), timed(True)  # warm-up (graph capture, first launches)
+    fwd = min(timed(False)[0] for _ in range(3))
+    grad = min(timed(True)[0] for _ in range(3))
+    log(f"forward {fwd * 1e3:.1f} ms, gradient ({len(names)} directions) {grad * 1e3:.1f} ms")
+    setup = dict(xyz=w["xyz"], r=w["r"], mat=mat, box=w["box"], cfg=cfg, steps=steps)
+    s, J = autograd.run_window(setup, names, theta, sens=True)
+    # points before the first platen pause (a controller event the tangents
+    # do not differentiate)
+    c = Context(w["xyz"], w["r"], mat, w["box"])
+    c.build_bonds()
+    c.set_test(cfg)
+    npts = 0
+    while c.test_state().steps_stop == 0 and c.test_state().step < steps:
+        c.step(cfg.dnsave)
+        if c.test_state().steps_stop == 0:
+            npts += 1
+    c.close()
+    tan = 0.5 * (J[:, :, 1] - J[:, :, 0])
+    errs = {}
+    for k, name in enumerate(names):
+        h = 1e-2 * abs(theta[k])
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        a, _ = autograd.run_window(setup, names, tp, sens=False)
+        b, _ = autograd.run_window(setup, names, tm, sens=False)
+        fd = (0.5 * (a[:, 1] - a[:, 0]) - 0.5 * (b[:, 1] - b[:, 0])) / (2 * h)
+        sel = slice(1, npts)
+        rel = np.abs(tan[sel, k] - fd[sel]) / np.maximum(np.abs(fd[sel]), 1e-300)
+        errs[name] = {"median": float(np.median(rel)), "max": float(rel.max())}
+    out = {
+        "metric": "gradient cost / forward pre-peak run (20k spheres, 2000 steps, 3 directions)",
+        "value": grad / fwd,
+        "unit": "x forward",
+        "n_gpus": 1,
+        "steps": steps,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator, FIRE-relaxed on the GPU)",
+        "config": {"workload": f"BPM uniaxial pre-peak window, {n} spheres r~U[0.20,0.30] mm, "
+                               f"{steps} steps to 0.1% strain",
+                   "directions": names, "forward_ms": fwd * 1e3, "gradient_ms": grad * 1e3,
+                   "fd_points": npts - 1,
+                   "tangent_vs_central_difference_1e-2": errs},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,12 +267,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="measure a batch of this many samples against sequential runs and exit")
+    ap.add_argument("--grad", action="store_true",
+                    help="measure the gradient run against the forward run and exit")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.batch > 0:
         batch_bench(args.batch, args.steps, args.warmup)
+        return
+    if args.grad:
+        grad_bench()
         return
     if args.copy_probe > 0:
         import torch
