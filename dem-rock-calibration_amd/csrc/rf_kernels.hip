@@ -914,46 +914,67 @@ __global__ void k_prune_initial(DevState s, DevParams p, Ctrl* ctrl) {
 // is not a current row partner.  Particles at the hidden point neither have
 // nor are candidates: they sit at one point outside the box (rho = 0 never
 // links, d_ReConstructInteractionListbpm's 1e-10 < rr) and never move again.
-__global__ void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.n) return;
+// Eight lanes per particle, lane l scanning stencil cells l, l+8, l+16, l+24:
+// a counting pass, a prefix over the eight counts, and a writing pass, so the
+// list order is fixed (lane by lane, cell by cell, sorted order in a cell).
+// The order of a list never changes a result: new pairs are ranked by
+// stencil cell and id when they are appended.
+__global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
+    const int l = threadIdx.x & (kLanes - 1);
+    const int64_t i = (int64_t)blockIdx.x * (256 / kLanes) + (threadIdx.x >> 3);
+    const bool valid = i < p.n;
+    const int64_t ic = valid ? i : 0;
     const double4* pos = s.pos[ctrl->cur];
-    const double4 pi = pos[i];
-    const bool hidden_i = (pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]);
-    const int64_t key = s.ckey[i];
+    const double4 pi = pos[ic];
+    const bool skip = !valid | ((pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]));
+    const int64_t key = s.ckey[ic];
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t C = p.C;
-    const uint32_t occ = s.occ[i];
+    const uint32_t occ = s.occ[ic];
+    auto hit = [&](int32_t j) -> bool {
+        if (j == (int32_t)ic) return false;
+        const double4 pj = pos[j];
+        if ((pj.x == p.hid[0]) & (pj.y == p.hid[1]) & (pj.z == p.hid[2])) return false;
+        const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        if (!(ex * ex + ey * ey + ez * ez < p.cand_r2)) return false;
+        bool partner = false;  // current partners hold a slot already
+        for (uint32_t m = occ; m && !partner; m &= m - 1u)
+            partner = s.nbr[ic * p.S + __builtin_ctz(m)] == j;
+        return !partner;
+    };
+    auto cell_of = [&](int c) {
+        return key + (c % 3 - 1) + ((c / 3) % 3 - 1) * nx + (c / 9 - 1) * nxy;
+    };
     int cnt = 0;
-    bool over = false;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int64_t cc = key + dx + dy * nx + dz * nxy;
-                const int32_t st = s.cstart[cc], ct = s.ccount[cc];
-                for (int32_t q = 0; q < ct && !hidden_i; ++q) {
-                    const int32_t j = s.sorted[st + q];
-                    if (j == (int32_t)i) continue;
-                    const double4 pj = pos[j];
-                    if ((pj.x == p.hid[0]) & (pj.y == p.hid[1]) & (pj.z == p.hid[2])) continue;
-                    const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-                    if (!(ex * ex + ey * ey + ez * ez < p.cand_r2)) continue;
-                    bool partner = false;  // current partners hold a slot already
-                    for (uint32_t m = occ; m && !partner; m &= m - 1u)
-                        partner = s.nbr[i * p.S + __builtin_ctz(m)] == j;
-                    if (partner) continue;
-                    if (cnt >= C) {
-                        over = true;
-                        continue;
-                    }
-                    s.cand[i * C + cnt] = j;
-                    s.cslot[i * C + cnt] = -1;
-                    ++cnt;
-                }
-            }
-    if (over) atomicOr(&ctrl->err, kErrCandOverflow);
-    s.ncand[i] = cnt;
-    s.rbuild[i] = pi;
+    for (int c = l; c < 27 && !skip; c += kLanes) {
+        const int64_t cc = cell_of(c);
+        const int32_t st = s.cstart[cc], ct = s.ccount[cc];
+        for (int32_t q = 0; q < ct; ++q) cnt += hit(s.sorted[st + q]) ? 1 : 0;
+    }
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < kLanes; off <<= 1) {
+        const int t = __shfl_up(incl, off, kLanes);
+        if (l >= off) incl += t;
+    }
+    const int total = __shfl(incl, kLanes - 1, kLanes);
+    int w = incl - cnt;
+    for (int c = l; c < 27 && !skip && w < C; c += kLanes) {
+        const int64_t cc = cell_of(c);
+        const int32_t st = s.cstart[cc], ct = s.ccount[cc];
+        for (int32_t q = 0; q < ct && w < C; ++q) {
+            const int32_t j = s.sorted[st + q];
+            if (!hit(j)) continue;
+            s.cand[ic * C + w] = j;
+            s.cslot[ic * C + w] = -1;
+            ++w;
+        }
+    }
+    if (valid && l == 0) {
+        if (total > C) atomicOr(&ctrl->err, kErrCandOverflow);
+        s.ncand[i] = total < C ? total : (int32_t)C;
+        s.rbuild[i] = pi;
+    }
 }
 
 // Per-bond strengths drawn on the host, scattered into the bond records.
@@ -1047,8 +1068,8 @@ hipError_t launch_prune_initial(const DevState& s, const DevParams& p, Ctrl* ctr
 
 hipError_t launch_build_candidates(const DevState& s, const DevParams& p, Ctrl* ctrl,
                                    hipStream_t st) {
-    hipLaunchKernelGGL(k_build_candidates, dim3(grid_for(p.n, 128)), dim3(128), 0, st, s, p,
-                       ctrl);
+    hipLaunchKernelGGL(k_build_candidates, dim3(grid_for(p.n * kLanes, 256)), dim3(256), 0, st, s,
+                       p, ctrl);
     return hipGetLastError();
 }
 

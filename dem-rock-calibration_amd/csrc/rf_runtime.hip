@@ -99,17 +99,24 @@ struct rf_ctx {
     std::vector<void*> allocs;
     int32_t chunk = 256;
     int32_t timing = 0;  // bracket every timing-th step launch with events (0: off)
-    // A captured chunk of `graph_steps` (step, controller) launch pairs,
-    // replayed while the constants it captured are unchanged.
+    // Full chunks of (step, controller) launch pairs run as captured graphs,
+    // replayed while the constants they captured are unchanged.
     bool use_graph = true;
     // sensitivity runs
     bool sens_on = false;
     int32_t sens_np = 0;
     int64_t sens_curve0 = 0;
     rf::SensState sens{};
-    hipGraphExec_t graph_exec = nullptr;
-    int64_t graph_steps = 0;
-    rf::DevParams graph_p{};
+    // Captured chunks by length: the loop shortens its chunks while list
+    // rebuilds come often (a rebuild halts the rest of a chunk, whose
+    // launches then do nothing) and lengthens them again after full chunks.
+    struct Graph {
+        int64_t steps = 0;
+        hipGraphExec_t exec = nullptr;
+        rf::DevParams p{};
+    };
+    std::vector<Graph> graphs;
+    int32_t chunk_eff = 256;
     std::vector<hipEvent_t> ev;
     double last_kernel_ms = 0.0, last_total_ms = 0.0;
     int64_t last_steps = 0, last_timed = 0;
@@ -369,7 +376,8 @@ rf_status rf_destroy(rf_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+    for (auto& g : c->graphs)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     for (void* ptr : c->allocs) hipFree(ptr);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -402,33 +410,52 @@ rf_status device_error(rf_ctx* c) {
 
 // Launch `k` (step, controller) pairs as one captured graph.  The kernels
 // read the step's buffers and halt flag from the device control block, so the
-// same graph serves every chunk; it is captured again only when the
-// constants passed by value (DevParams) or the chunk length change.
+// same graph serves every chunk of its length; it is captured again only when
+// the constants passed by value (DevParams) change.
 rf_status graph_chunk(rf_ctx* c, int64_t k) {
-    if (!c->graph_exec || c->graph_steps != k ||
-        std::memcmp(&c->graph_p, &c->p, sizeof(rf::DevParams)) != 0) {
-        if (c->graph_exec) {
-            HIP_TRY(hipGraphExecDestroy(c->graph_exec));
-            c->graph_exec = nullptr;
+    rf_ctx::Graph* g = nullptr;
+    for (auto& q : c->graphs)
+        if (q.steps == k) g = &q;
+    if (!g) {
+        c->graphs.push_back(rf_ctx::Graph{});
+        g = &c->graphs.back();
+        g->steps = k;
+    }
+    if (!g->exec || std::memcmp(&g->p, &c->p, sizeof(rf::DevParams)) != 0) {
+        if (g->exec) {
+            HIP_TRY(hipGraphExecDestroy(g->exec));
+            g->exec = nullptr;
         }
-        hipGraph_t g = nullptr;
+        hipGraph_t gr = nullptr;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int64_t s = 0; s < k; ++s) {
             rf::launch_step(c->d, c->p, c->ctrl, c->stream);
             rf::launch_controller(c->d, c->p, c->ctrl, c->stream);
         }
-        HIP_TRY(hipStreamEndCapture(c->stream, &g));
-        const hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        HIP_TRY(hipStreamEndCapture(c->stream, &gr));
+        const hipError_t e = hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
         if (e != hipSuccess) {
-            c->graph_exec = nullptr;
+            g->exec = nullptr;
             return fail(RF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
         }
-        c->graph_steps = k;
-        c->graph_p = c->p;
+        g->p = c->p;
     }
-    HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    HIP_TRY(hipGraphLaunch(g->exec, c->stream));
     return RF_OK;
+}
+
+// Next chunk length: after a chunk cut short by a list rebuild, the power of
+// two at or above the steps it ran (at least 8); after a full chunk, twice
+// the length, up to the set chunk.
+void adapt_chunk(rf_ctx* c, int64_t k, int64_t executed, bool rebuilt) {
+    if (rebuilt && executed < k) {
+        int64_t q = 8;
+        while (q < executed) q <<= 1;
+        c->chunk_eff = (int32_t)std::min<int64_t>(q, c->chunk);
+    } else if (executed == k && k == c->chunk_eff) {
+        c->chunk_eff = (int32_t)std::min<int64_t>(2 * (int64_t)c->chunk_eff, c->chunk);
+    }
 }
 
 void set_skin(rf_ctx* c, double skin) {
@@ -610,6 +637,7 @@ rf_status rf_set_graph(rf_ctx* c, int32_t enable) {
 rf_status rf_set_chunk(rf_ctx* c, int32_t steps) {
     if (!c || steps <= 0) return fail(RF_ERR_ARG, "bad chunk");
     c->chunk = steps;
+    c->chunk_eff = steps;
     return RF_OK;
 }
 
@@ -635,7 +663,7 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
     double kernel_ms = 0.0;
     int64_t timed = 0;
     while (remaining > 0 && !c->hctrl.done) {
-        const int64_t k = std::min<int64_t>(remaining, c->chunk);
+        const int64_t k = std::min<int64_t>(remaining, c->chunk_eff);
         if (c->timing && (int64_t)c->ev.size() < 2 * k) {
             while ((int64_t)c->ev.size() < 2 * k) {
                 hipEvent_t e;
@@ -646,7 +674,7 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         const int64_t before = c->hctrl.step;
         // A full chunk runs as a captured graph; with timing on, its last step
         // is launched on its own between two events (one sample per chunk).
-        const bool graphed = c->use_graph && k == c->chunk && k > 1;
+        const bool graphed = c->use_graph && k == c->chunk_eff && k > 1;
         if (graphed) {
             st = graph_chunk(c, c->timing ? k - 1 : k);
             if (st) return st;
@@ -686,7 +714,9 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         }
         remaining -= executed;
         if (c->hctrl.err) return device_error(c);
-        if (c->hctrl.rebuild) {
+        const bool rebuilt = c->hctrl.rebuild != 0;
+        adapt_chunk(c, k, executed, rebuilt);
+        if (rebuilt) {
             st = rebuild(c);
             if (st) return st;
             if (c->hctrl.err) return device_error(c);
@@ -1393,7 +1423,7 @@ rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
     const int64_t start = c->hctrl.step;
     int64_t remaining = nsteps;
     while (remaining > 0 && !c->hctrl.done) {
-        const int64_t k = std::min<int64_t>(remaining, c->chunk);
+        const int64_t k = std::min<int64_t>(remaining, c->chunk_eff);
         const int64_t before = c->hctrl.step;
         for (int64_t s = 0; s < k; ++s) {
             HIP_TRY(rf::launch_sens_step(c->d, c->p, c->sens, c->ctrl, c->stream));
@@ -1407,6 +1437,7 @@ rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
             return fail(RF_ERR_STATE, "sensitivity window: a touching pair has no slot, two "
                                       "particles coincide, or a particle left the box");
         if (c->hctrl.err) return device_error(c);
+        adapt_chunk(c, k, executed, c->hctrl.rebuild != 0);
         if (c->hctrl.rebuild) {
             st = rebuild(c);
             if (st) return st;

@@ -122,6 +122,7 @@ __global__ __launch_bounds__(kSensBlock) void k_sens_step(DevState s, DevParams 
         // four partial sums are live.
         auto tree_sum = [&](auto&& lane_sum, D (&total)[6]) {
             D hold[6], p1[6], q1[6], acc[6];
+#pragma unroll 1
             for (int idx = 0; idx < kLanes; ++idx) {
                 const int l = ((idx & 1) << 2) | (idx & 2) | ((idx >> 2) & 1);  // 0,4,2,6,1,5,3,7
 #pragma unroll
@@ -145,6 +146,7 @@ __global__ __launch_bounds__(kSensBlock) void k_sens_step(DevState s, DevParams 
         // bonds in creation order; entry b goes to lane b % 8
         const int nbd = s.nbond[i];
         auto bond_lane = [&](int l, D (&acc)[6]) {
+#pragma unroll 1
             for (int b = l; b < nbd && b < B; b += kLanes) {
                 const int ks = s.bslot[i * B + b];
                 if (ks < 0 || !((bnd >> ks) & 1u)) continue;
@@ -188,6 +190,7 @@ __global__ __launch_bounds__(kSensBlock) void k_sens_step(DevState s, DevParams 
         const uint32_t cmask = occ & ~bnd;
         uint32_t ovl = 0u;
         auto contact_lane = [&](int l, D (&acc)[6]) {
+#pragma unroll 1
             for (int k = l; k < 32; k += kLanes) {
                 if (!((cmask >> k) & 1u)) continue;
                 const int64_t j = s.nbr[i * S + k];
