@@ -118,6 +118,11 @@ struct Ctrl {
 // of (E, G, hm_E, hm_G, mu, mu_r, p_A, E_wall, G_wall, bond radius ratio,
 // bond radius) along direction k.
 struct SensState {
+    int32_t nl;        // lanes per particle = parameter directions in use
+    int32_t pad_;
+    double* pp;        // per particle: bottom, top platen force (values)
+    int32_t* pnb;      // per particle: bonds left
+    double* tpp;       // per particle and direction: bottom, top tangents
     double4* tpos[2];
     double4* tvel[2];
     double4* tang[2];

@@ -1378,6 +1378,9 @@ rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
         }
         rf_status st = c->alloc(&t.tbrec, K * n * c->B * 12);
         if (!st) st = c->alloc(&t.tot, K * n * c->S);
+        if (!st) st = c->alloc(&t.pp, 2 * n);
+        if (!st) st = c->alloc(&t.pnb, n);
+        if (!st) st = c->alloc(&t.tpp, 2 * K * n);
         if (!st) st = c->alloc(&t.tpart_b, K * c->p.nblk);
         if (!st) st = c->alloc(&t.tpart_t, K * c->p.nblk);
         if (st) return st;
@@ -1408,6 +1411,7 @@ rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
             for (int u = 0; u < 11; ++u) t.dmat[k][u] = 0.0;
         }
     }
+    t.nl = nparam;
     c->sens_np = nparam;
     c->sens_curve0 = c->hctrl.stepsave;
     HIP_TRY(hipStreamSynchronize(c->stream));

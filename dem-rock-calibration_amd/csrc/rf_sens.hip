@@ -4,15 +4,16 @@
 // Every quantity of the step carries its value and its derivative along a
 // parameter direction (Dual<1>, rf_physics.h), seeded by the derivatives of
 // the material constants (E, G, hm_E, hm_G, mu, mu_r, wall moduli, bond
-// radius ratio) with respect to the chosen parameter.  kSensK adjacent lanes
-// of a wave work on one particle, lane k carrying direction k: each lane
-// recomputes the value (the same IEEE operations in the same order, so all
-// lanes hold the same values) and one tangent.  Against one lane carrying
-// all kSensK tangents this trades repeated value arithmetic for a fourth of
-// the registers and four times the lanes.  Lane 0 of a particle writes the
-// values, every lane its tangent slice; tangent arrays are laid out
-// [entity][direction] so the lanes of a particle read and write one
-// contiguous run.  Per-particle sums follow the step kernel's lane order, so
+// radius ratio) with respect to the chosen parameter.  As many adjacent
+// lanes of a wave as there are directions (1..kSensK) work on one particle,
+// lane k carrying direction k: each lane recomputes the value (the same IEEE
+// operations in the same order, so all lanes hold the same values) and one
+// tangent.  Against one lane carrying all tangents this trades repeated
+// value arithmetic for a fraction of the registers and more lanes.  Lane 0
+// of a particle writes the values, every lane its tangent slice; tangent
+// arrays are laid out [entity][kSensK] so the lanes of a particle read and
+// write one contiguous run.  Platen forces go out per particle and a second
+// kernel adds them per 32 particles in the value controller's tree order.  Per-particle sums follow the step kernel's lane order, so
 // the values are the step kernel's bit for bit.  The slot table is held
 // fixed: pairs crossing acut carry no force either way, and a touching pair
 // without a slot ends the window with an error.  The step size dt is a
@@ -33,7 +34,13 @@ using D = Dual<1>;
 namespace {
 
 constexpr int kSensBlock = 256;
-constexpr int kSensPB = kSensBlock / kSensK;  // particles per block
+
+// Minimum waves per SIMD the register allocation must allow (1: no limit,
+// the kernel keeps everything in registers; 2 halves the registers at the
+// price of spills).
+#ifndef RF_SENS_WAVES
+#define RF_SENS_WAVES 1
+#endif
 
 __device__ __forceinline__ D mk(double v, double t) {
     D r;
@@ -64,17 +71,16 @@ __device__ __forceinline__ MatT<D> mat_dual(const DevParams& p, const SensState&
 
 }  // namespace
 
-__global__ __launch_bounds__(kSensBlock) void k_sens_step(DevState s, DevParams p, SensState t,
+__global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevState s, DevParams p, SensState t,
                                                           Ctrl* __restrict__ ctrl) {
-    __shared__ double sh_b[kSensPB], sh_t[kSensPB];
-    __shared__ double th_b[kSensK][kSensPB], th_t[kSensK][kSensPB];
-    __shared__ int sh_nb[kSensPB];
     if (ctrl->halt) return;
-    const int tid = threadIdx.x;
-    const int dir = tid % kSensK, pl = tid / kSensK;
+    const int nl = t.nl;
+    const int ppw = 64 / nl;  // particles per wave
+    const int lane = threadIdx.x & 63;
+    const int dir = lane % nl, pw = lane / nl;
     const bool lead = dir == 0;
-    const int64_t i = (int64_t)blockIdx.x * kSensPB + pl;
-    const bool valid = i < p.n;
+    const int64_t i = ((int64_t)blockIdx.x * (kSensBlock / 64) + (threadIdx.x >> 6)) * ppw + pw;
+    const bool valid = pw < ppw && i < p.n;
     const int cur = ctrl->cur, nxt = cur ^ 1;
     const double top = ctrl->top, bottom = ctrl->bottom, vw = ctrl->vw;
     const int S = p.S, C = p.C, B = p.B;
@@ -294,37 +300,59 @@ __global__ __launch_bounds__(kSensBlock) void k_sens_step(DevState s, DevParams 
         }
         nb_out = __popc(bnd);
     }
-    // partials per 32 particles (the value controller's grouping)
-    if (lead) {
-        sh_b[pl] = pb.v;
-        sh_t[pl] = pt.v;
-        sh_nb[pl] = nb_out;
+    if (valid) {
+        if (lead) {
+            t.pp[2 * i] = pb.v;
+            t.pp[2 * i + 1] = pt.v;
+            t.pnb[i] = nb_out;
+        }
+        t.tpp[(i * kSensK + dir) * 2] = pb.d[0];
+        t.tpp[(i * kSensK + dir) * 2 + 1] = pt.d[0];
     }
-    th_b[dir][pl] = pb.d[0];
-    th_t[dir][pl] = pt.d[0];
+}
+
+// Platen forces and bond counts per 32 particles, in the value controller's
+// order: the tree v[g] += v[g+off], off = 16..1, with -0.0 past the last
+// particle; tangents the same way per direction.
+__global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t, DevState s,
+                                                       Ctrl* __restrict__ ctrl) {
+    __shared__ double vb[256], vt[256];
+    __shared__ double db[kSensK][256], dt_[kSensK][256];
+    __shared__ int nb[256];
+    if (ctrl->halt) return;
+    const int tid = threadIdx.x, g = tid & 31;
+    const int64_t grp = (int64_t)blockIdx.x * 8 + (tid >> 5);
+    const int64_t i = grp * 32 + g;
+    const bool in = i < p.n;
+    const int nl = t.nl;
+    vb[tid] = in ? t.pp[2 * i] : -0.0;
+    vt[tid] = in ? t.pp[2 * i + 1] : -0.0;
+    nb[tid] = in ? t.pnb[i] : 0;
+    for (int k = 0; k < nl; ++k) {
+        db[k][tid] = in ? t.tpp[(i * kSensK + k) * 2] : -0.0;
+        dt_[k][tid] = in ? t.tpp[(i * kSensK + k) * 2 + 1] : -0.0;
+    }
     __syncthreads();
-    const int g = pl & 31;
     for (int off = 16; off >= 1; off >>= 1) {
         if (g < off) {
-            if (lead) {
-                sh_b[pl] = sh_b[pl] + sh_b[pl + off];
-                sh_t[pl] = sh_t[pl] + sh_t[pl + off];
-                sh_nb[pl] += sh_nb[pl + off];
+            vb[tid] = vb[tid] + vb[tid + off];
+            vt[tid] = vt[tid] + vt[tid + off];
+            nb[tid] += nb[tid + off];
+            for (int k = 0; k < nl; ++k) {
+                db[k][tid] = db[k][tid] + db[k][tid + off];
+                dt_[k][tid] = dt_[k][tid] + dt_[k][tid + off];
             }
-            th_b[dir][pl] = th_b[dir][pl] + th_b[dir][pl + off];
-            th_t[dir][pl] = th_t[dir][pl] + th_t[dir][pl + off];
         }
         __syncthreads();
     }
-    const int64_t grp = (int64_t)blockIdx.x * (kSensPB / 32) + (pl >> 5);
     if (g == 0 && grp < p.nblk) {
-        if (lead) {
-            s.part_b[grp] = sh_b[pl];
-            s.part_t[grp] = sh_t[pl];
-            s.part_bonds[grp] = sh_nb[pl];
+        s.part_b[grp] = vb[tid];
+        s.part_t[grp] = vt[tid];
+        s.part_bonds[grp] = nb[tid];
+        for (int k = 0; k < kSensK; ++k) {
+            t.tpart_b[k * p.nblk + grp] = k < nl ? db[k][tid] : 0.0;
+            t.tpart_t[k * p.nblk + grp] = k < nl ? dt_[k][tid] : 0.0;
         }
-        t.tpart_b[dir * p.nblk + grp] = th_b[dir][pl];
-        t.tpart_t[dir * p.nblk + grp] = th_t[dir][pl];
     }
 }
 
@@ -367,8 +395,11 @@ __global__ __launch_bounds__(256) void k_sens_ctrl(DevParams p, SensState t, Ctr
 
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
                             hipStream_t st) {
-    hipLaunchKernelGGL(k_sens_step, dim3((unsigned)((p.n + kSensPB - 1) / kSensPB)), dim3(kSensBlock),
-                       0, st, s, p, t, ctrl);
+    const int64_t per_block = (int64_t)(kSensBlock / 64) * (64 / t.nl);
+    hipLaunchKernelGGL(k_sens_step, dim3((unsigned)((p.n + per_block - 1) / per_block)),
+                       dim3(kSensBlock), 0, st, s, p, t, ctrl);
+    hipLaunchKernelGGL(k_sens_partials, dim3((unsigned)((p.nblk + 7) / 8)), dim3(256), 0, st, p, t,
+                       s, ctrl);
     hipLaunchKernelGGL(k_sens_ctrl, dim3(1), dim3(256), 0, st, p, t, ctrl);
     return hipGetLastError();
 }
