@@ -931,16 +931,30 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t C = p.C;
     const uint32_t occ = s.occ[ic];
-    auto hit = [&](int32_t j) -> bool {
-        if (j == (int32_t)ic) return false;
-        const double4 pj = pos[j];
-        if ((pj.x == p.hid[0]) & (pj.y == p.hid[1]) & (pj.z == p.hid[2])) return false;
-        const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-        if (!(ex * ex + ey * ey + ez * ez < p.cand_r2)) return false;
-        bool partner = false;  // current partners hold a slot already
-        for (uint32_t m = occ; m && !partner; m &= m - 1u)
-            partner = s.nbr[ic * p.S + __builtin_ctz(m)] == j;
-        return !partner;
+    // bit u of the result: entry q + u of the cell is a candidate (four
+    // entries per round, their loads issued together)
+    auto hits4 = [&](int32_t st, int32_t ct, int32_t q, int32_t (&j)[4]) -> uint32_t {
+        double4 pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = q + u < ct ? s.sorted[st + q + u] : (int32_t)ic;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pj[u] = pos[j[u]];
+        uint32_t m4 = 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double ex = pj[u].x - pi.x, ey = pj[u].y - pi.y, ez = pj[u].z - pi.z;
+            const bool hid = (pj[u].x == p.hid[0]) & (pj[u].y == p.hid[1]) & (pj[u].z == p.hid[2]);
+            const bool in = (j[u] != (int32_t)ic) & !hid & (ex * ex + ey * ey + ez * ez < p.cand_r2);
+            m4 |= (uint32_t)in << u;
+        }
+        for (uint32_t b = m4; b; b &= b - 1u) {  // current partners hold a slot already
+            const int u = __builtin_ctz(b);
+            bool partner = false;
+            for (uint32_t m = occ; m && !partner; m &= m - 1u)
+                partner = s.nbr[ic * p.S + __builtin_ctz(m)] == j[u];
+            if (partner) m4 &= ~(1u << u);
+        }
+        return m4;
     };
     auto cell_of = [&](int c) {
         return key + (c % 3 - 1) + ((c / 3) % 3 - 1) * nx + (c / 9 - 1) * nxy;
@@ -949,7 +963,10 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     for (int c = l; c < 27 && !skip; c += kLanes) {
         const int64_t cc = cell_of(c);
         const int32_t st = s.cstart[cc], ct = s.ccount[cc];
-        for (int32_t q = 0; q < ct; ++q) cnt += hit(s.sorted[st + q]) ? 1 : 0;
+        for (int32_t q = 0; q < ct; q += 4) {
+            int32_t j[4];
+            cnt += __popc(hits4(st, ct, q, j));
+        }
     }
     int incl = cnt;
 #pragma unroll
@@ -962,12 +979,14 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     for (int c = l; c < 27 && !skip && w < C; c += kLanes) {
         const int64_t cc = cell_of(c);
         const int32_t st = s.cstart[cc], ct = s.ccount[cc];
-        for (int32_t q = 0; q < ct && w < C; ++q) {
-            const int32_t j = s.sorted[st + q];
-            if (!hit(j)) continue;
-            s.cand[ic * C + w] = j;
-            s.cslot[ic * C + w] = -1;
-            ++w;
+        for (int32_t q = 0; q < ct && w < C; q += 4) {
+            int32_t j[4];
+            for (uint32_t b = hits4(st, ct, q, j); b && w < C; b &= b - 1u) {
+                const int u = __builtin_ctz(b);
+                s.cand[ic * C + w] = j[u];
+                s.cslot[ic * C + w] = -1;
+                ++w;
+            }
         }
     }
     if (valid && l == 0) {
