@@ -151,6 +151,7 @@ def batch_bench(count, steps, warmup):
     device_sync()
     t_bat = time.perf_counter() - t1
     bat_steps = sum(done)
+    order, tune_ms = b.tuning()
     pst_bat = sum(w["n"] * d for w, d in zip(ws, done)) / t_bat
     pst_seq = total_n * (seq_steps / count) / t_seq
     log(f"batched {t_bat:.3f}s ({bat_steps} sample-steps), sequential {t_seq:.3f}s "
@@ -166,6 +167,8 @@ def batch_bench(count, steps, warmup):
                        "kind": "same samples one after another on the GPU"},
         "speedup": pst_bat / pst_seq,
         "sample_steps_per_s": {"batched": bat_steps / t_bat, "sequential": seq_steps / t_seq},
+        "block_order": {"chosen": ["by sample", "interleaved", "largest first"][order]
+                        if order >= 0 else None, "ms_per_step": tune_ms},
     }))
 
 

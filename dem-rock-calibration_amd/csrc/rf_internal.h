@@ -137,14 +137,11 @@ struct SensState {
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
                             hipStream_t st);
 
-// One sample of a batched launch: its state, constants, control block and
-// the first logical block of the step kernel that belongs to it.
+// One sample of a batched launch: its state, constants and control block.
 struct BatchEntry {
     DevState s;
     DevParams p;
     Ctrl* ctrl;
-    int32_t blk0;
-    int32_t pad;
 };
 
 // Sets the text rf_last_error() returns and passes `st` through
@@ -164,7 +161,8 @@ hipError_t launch_set_strengths(BondRec* brec, const double* sc, const double* t
                                 hipStream_t st);
 hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
-hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample, int32_t nblk_total,
+hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample,
+                             const int32_t* blk_local, int32_t nblk_total,
                              hipStream_t st);
 hipError_t launch_controller_batch(const BatchEntry* e, int32_t count, hipStream_t st);
 hipError_t launch_counters(const DevState& s, const DevParams& p, Ctrl* ctrl, int64_t* out3,

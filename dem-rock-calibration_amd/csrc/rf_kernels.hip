@@ -624,14 +624,16 @@ __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevPar
     step_body(s, p, ctrl, xcd_block((int)blockIdx.x, (int)gridDim.x));
 }
 
-// Many samples in one launch: logical block L belongs to sample
-// blk_sample[L] and is that sample's block L - blk0, so every sample runs
-// exactly as it would alone.
+// Many samples in one launch: logical block L runs block blk_local[L] of
+// sample blk_sample[L], so every sample runs exactly as it would alone; the
+// host chooses the order of the table (the batch autotuner), which never
+// changes a result.
 __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step_batch(
-    const BatchEntry* __restrict__ e, const int32_t* __restrict__ blk_sample) {
+    const BatchEntry* __restrict__ e, const int32_t* __restrict__ blk_sample,
+    const int32_t* __restrict__ blk_local) {
     const int L = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const BatchEntry& b = e[blk_sample[L]];
-    step_body(b.s, b.p, b.ctrl, L - b.blk0);
+    step_body(b.s, b.p, b.ctrl, blk_local[L]);
 }
 
 // -------------------------------------------------------- controller ------
@@ -1118,10 +1120,10 @@ hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, 
     return hipGetLastError();
 }
 
-hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample, int32_t nblk_total,
-                             hipStream_t st) {
+hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample,
+                             const int32_t* blk_local, int32_t nblk_total, hipStream_t st) {
     hipLaunchKernelGGL(k_step_batch, dim3((unsigned)nblk_total), dim3(kBlock), 0, st, e,
-                       blk_sample);
+                       blk_sample, blk_local);
     return hipGetLastError();
 }
 

@@ -1112,6 +1112,18 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
 // blocks of the shared launch.  A sample stops at its own step target
 // (Ctrl::stop_at), at the end of its test, or for a list rebuild, which the
 // host then runs for that sample alone.
+//
+// Autotuning: the order of the block table is free (a block's work depends
+// only on its sample and its local index).  The first steps of a batch run
+// short chunks under each candidate order, timed with events on the batch
+// stream, and the fastest order is kept: samples one after another, blocks
+// of all samples interleaved, or samples one after another largest first.
+// Full chunks replay as captured graphs (their kernels read the tables and
+// entries from device memory, so one graph per chunk length serves every
+// order), and chunks shorten while list rebuilds come often.
+
+constexpr int kBatchOrders = 3;
+constexpr int kTuneSteps = 32;
 
 struct rf_batch {
     std::vector<rf_ctx*> c;
@@ -1119,28 +1131,108 @@ struct rf_batch {
     hipStream_t stream = nullptr;
     rf::BatchEntry* d_e = nullptr;
     int32_t* d_map = nullptr;
+    int32_t* d_loc = nullptr;
     int32_t nblk_total = 0;
     int32_t chunk = 256;
+    int32_t chunk_eff = 256;
+    int32_t order = -1;      // table order in use
+    int32_t chosen = -1;     // order picked by the tuner (-1: not yet)
+    int32_t tuning = 0;      // next order to time
+    double tune_ms[kBatchOrders] = {0.0, 0.0, 0.0};
     double last_ms = 0.0;
+    struct Graph {
+        int64_t steps = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
 };
 
 namespace {
 
 rf_status batch_upload(rf_batch* b) {
     std::vector<rf::BatchEntry> e(b->c.size());
-    int32_t off = 0;
     for (size_t k = 0; k < b->c.size(); ++k) {
         const rf_ctx* c = b->c[k];
         e[k].s = c->d;
         e[k].p = c->p;
         e[k].ctrl = c->ctrl;
-        e[k].blk0 = off;
-        e[k].pad = 0;
-        off += c->p.nblk;
     }
     HIP_TRY(hipMemcpyAsync(b->d_e, e.data(), e.size() * sizeof(rf::BatchEntry),
                            hipMemcpyHostToDevice, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return RF_OK;
+}
+
+// Block table for an order (see rf_batch).
+rf_status batch_order(rf_batch* b, int order) {
+    if (order == b->order) return RF_OK;
+    const size_t count = b->c.size();
+    std::vector<int32_t> smp, loc;
+    smp.reserve(b->nblk_total);
+    loc.reserve(b->nblk_total);
+    std::vector<size_t> seq(count);
+    for (size_t k = 0; k < count; ++k) seq[k] = k;
+    if (order == 2)
+        std::stable_sort(seq.begin(), seq.end(), [&](size_t u, size_t v) {
+            return b->c[u]->p.nblk > b->c[v]->p.nblk;
+        });
+    if (order == 1) {
+        std::vector<int32_t> next(count, 0);
+        for (bool more = true; more;) {
+            more = false;
+            for (size_t k = 0; k < count; ++k)
+                if (next[k] < b->c[k]->p.nblk) {
+                    smp.push_back((int32_t)k);
+                    loc.push_back(next[k]++);
+                    more = true;
+                }
+        }
+    } else {
+        for (size_t k : seq)
+            for (int32_t q = 0; q < b->c[k]->p.nblk; ++q) {
+                smp.push_back((int32_t)k);
+                loc.push_back(q);
+            }
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_map, smp.data(), smp.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_loc, loc.data(), loc.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->order = order;
+    return RF_OK;
+}
+
+// `k` (step, controller) launch pairs of the batch, as one captured graph
+// when k is the current full chunk length.
+rf_status batch_launch(rf_batch* b, int64_t k, bool graphed) {
+    const int32_t count = (int32_t)b->c.size();
+    if (!graphed) {
+        for (int64_t s = 0; s < k; ++s) {
+            HIP_TRY(rf::launch_step_batch(b->d_e, b->d_map, b->d_loc, b->nblk_total, b->stream));
+            HIP_TRY(rf::launch_controller_batch(b->d_e, count, b->stream));
+        }
+        return RF_OK;
+    }
+    rf_batch::Graph* g = nullptr;
+    for (auto& q : b->graphs)
+        if (q.steps == k) g = &q;
+    if (!g) {
+        hipGraph_t gr = nullptr;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int64_t s = 0; s < k; ++s) {
+            rf::launch_step_batch(b->d_e, b->d_map, b->d_loc, b->nblk_total, b->stream);
+            rf::launch_controller_batch(b->d_e, count, b->stream);
+        }
+        HIP_TRY(hipStreamEndCapture(b->stream, &gr));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (e != hipSuccess) return fail(RF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        b->graphs.push_back(rf_batch::Graph{k, ex});
+        g = &b->graphs.back();
+    }
+    HIP_TRY(hipGraphLaunch(g->exec, b->stream));
     return RF_OK;
 }
 
@@ -1185,18 +1277,15 @@ rf_status rf_batch_create(rf_ctx* const* ctxs, int32_t count, rf_batch** out) {
     if (e == hipSuccess) e = hipStreamCreate(&b->stream);
     if (e == hipSuccess) e = hipMalloc(&b->d_e, count * sizeof(rf::BatchEntry));
     if (e == hipSuccess) e = hipMalloc(&b->d_map, std::max<int64_t>(total, 1) * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_loc, std::max<int64_t>(total, 1) * sizeof(int32_t));
     if (e != hipSuccess) {
         rf_batch_destroy(b);
         return fail(RF_ERR_HIP, "batch allocation failed: %s", hipGetErrorString(e));
     }
-    std::vector<int32_t> map((size_t)total);
-    int64_t off = 0;
-    for (int32_t k = 0; k < count; ++k)
-        for (int32_t q = 0; q < b->c[k]->p.nblk; ++q) map[off++] = k;
-    e = hipMemcpy(b->d_map, map.data(), total * sizeof(int32_t), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
+    rf_status st = batch_order(b, 0);
+    if (st) {
         rf_batch_destroy(b);
-        return fail(RF_ERR_HIP, "batch upload failed: %s", hipGetErrorString(e));
+        return st;
     }
     *out = b;
     return RF_OK;
@@ -1208,6 +1297,9 @@ rf_status rf_batch_destroy(rf_batch* b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->d_e) hipFree(b->d_e);
     if (b->d_map) hipFree(b->d_map);
+    if (b->d_loc) hipFree(b->d_loc);
+    for (auto& g : b->graphs)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return RF_OK;
@@ -1230,27 +1322,40 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
     if (st) return st;
     st = batch_sync(b);
     if (st) return st;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr, ta = nullptr, tb = nullptr;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventCreate(&ta));
+    HIP_TRY(hipEventCreate(&tb));
     HIP_TRY(hipEventRecord(t0, b->stream));
     rf_status result = RF_OK;
     for (;;) {
         bool active = false;
         for (rf_ctx* c : b->c) active = active || !c->hctrl.halt;
         if (!active) break;
-        std::vector<int64_t> before(count);
-        for (size_t k = 0; k < count; ++k) before[k] = b->c[k]->hctrl.step;
-        for (int32_t s = 0; s < b->chunk; ++s) {
-            HIP_TRY(rf::launch_step_batch(b->d_e, b->d_map, b->nblk_total, b->stream));
-            HIP_TRY(rf::launch_controller_batch(b->d_e, (int32_t)count, b->stream));
+        const bool tune = b->chosen < 0;
+        if (tune) {
+            st = batch_order(b, b->tuning);
+            if (st) { result = st; break; }
         }
+        const int64_t k = tune ? kTuneSteps : b->chunk_eff;
+        std::vector<int64_t> before(count);
+        std::vector<char> idle(count);  // halted for good before the chunk
+        for (size_t q = 0; q < count; ++q) {
+            before[q] = b->c[q]->hctrl.step;
+            idle[q] = b->c[q]->hctrl.halt != 0;
+        }
+        HIP_TRY(hipEventRecord(ta, b->stream));
+        st = batch_launch(b, k, !tune && k > 1);
+        if (st) { result = st; break; }
+        HIP_TRY(hipEventRecord(tb, b->stream));
         st = batch_sync(b);
         if (st) { result = st; break; }
-        bool progress = false, rebuilt = false;
-        for (size_t k = 0; k < count && !result; ++k) {
-            rf_ctx* c = b->c[k];
-            progress = progress || c->hctrl.step != before[k];
+        bool progress = false, rebuilt = false, full = true;
+        for (size_t q = 0; q < count && !result; ++q) {
+            rf_ctx* c = b->c[q];
+            progress = progress || c->hctrl.step != before[q];
+            full = full && (idle[q] || c->hctrl.step - before[q] == k);
             if (c->hctrl.err) { result = device_error(c); break; }
             if (c->hctrl.rebuild) {
                 st = rebuild(c);
@@ -1260,6 +1365,29 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
             }
         }
         if (result) break;
+        if (tune && full) {  // a clean timing: every active sample ran every step
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ta, tb));
+            b->tune_ms[b->tuning] = ms / (double)k;
+            if (++b->tuning == kBatchOrders) {
+                int best = 0;
+                for (int o = 1; o < kBatchOrders; ++o)
+                    if (b->tune_ms[o] < b->tune_ms[best]) best = o;
+                b->chosen = best;
+                st = batch_order(b, best);
+                if (st) { result = st; break; }
+            }
+        } else if (!tune) {
+            if (rebuilt && !full) {
+                int64_t m = 0;
+                for (size_t q = 0; q < count; ++q) m = std::max(m, b->c[q]->hctrl.step - before[q]);
+                int64_t len = 8;
+                while (len < m) len <<= 1;
+                b->chunk_eff = (int32_t)std::min<int64_t>(len, b->chunk);
+            } else if (full) {
+                b->chunk_eff = (int32_t)std::min<int64_t>(2 * (int64_t)b->chunk_eff, b->chunk);
+            }
+        }
         if (rebuilt) {
             st = batch_upload(b);  // a rebuild may have narrowed the skin
             if (st) { result = st; break; }
@@ -1268,6 +1396,8 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
             break;
         }
     }
+    hipEventDestroy(ta);
+    hipEventDestroy(tb);
     HIP_TRY(hipEventRecord(t1, b->stream));
     HIP_TRY(hipEventSynchronize(t1));
     float ms = 0.f;
@@ -1297,6 +1427,14 @@ rf_status rf_batch_run(rf_batch* b, rf_test_state* out) {
         if (st) return st;
         if (out) fill_state(c, out + k);
     }
+    return RF_OK;
+}
+
+rf_status rf_batch_tuning(rf_batch* b, int32_t* order, double* ms_per_step) {
+    if (!b) return fail(RF_ERR_ARG, "null batch");
+    if (order) *order = b->chosen;
+    if (ms_per_step)
+        for (int o = 0; o < kBatchOrders; ++o) ms_per_step[o] = b->tune_ms[o];
     return RF_OK;
 }
 

@@ -29,7 +29,7 @@ EXPORTS = [
     "rf_fire_defaults", "rf_fire_relax", "rf_set_graph",
     "rf_sens_material_tangent", "rf_sens_begin", "rf_sens_step", "rf_sens_get_curve",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
-    "rf_linearity_deviation", "rf_curve_moduli",
+    "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning",
 ]
 
 
@@ -110,6 +110,8 @@ def lib():
     L.rf_batch_run.restype = st
     L.rf_batch_last_ms.argtypes = [vp, P(C.c_double)]
     L.rf_batch_last_ms.restype = st
+    L.rf_batch_tuning.argtypes = [vp, P(C.c_int32), P(C.c_double)]
+    L.rf_batch_tuning.restype = st
     L.rf_fire_defaults.argtypes = [P(abi.FireCfg)]
     L.rf_fire_defaults.restype = st
     L.rf_fire_relax.argtypes = [C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),

@@ -244,3 +244,10 @@ class Batch:
         ms = C.c_double()
         check(lib().rf_batch_last_ms(self.h, C.byref(ms)), "rf_batch_last_ms")
         return ms.value
+
+    def tuning(self):
+        """(chosen block order or -1, [ms per step of each order])."""
+        order = C.c_int32()
+        ms = (C.c_double * 3)()
+        check(lib().rf_batch_tuning(self.h, C.byref(order), ms), "rf_batch_tuning")
+        return order.value, list(ms)

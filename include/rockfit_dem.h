@@ -329,6 +329,13 @@ rf_status rf_batch_step(rf_batch* batch, int64_t nsteps, int64_t* done_steps);
 rf_status rf_batch_run(rf_batch* batch, rf_test_state* out);
 /* Wall time (ms) of the last rf_batch_step / rf_batch_run on the device. */
 rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
+/* The batch's launch autotuner (no reference counterpart): the first steps of
+ * a batch run in short chunks under each of three block orders (samples one
+ * after another, blocks of all samples interleaved, samples largest first),
+ * timed on the device, and the fastest order is kept.  The order never
+ * changes a result.  *order receives the chosen order (-1 before tuning),
+ * ms_per_step (3 entries, may be null) the measured step times. */
+rf_status rf_batch_tuning(rf_batch* batch, int32_t* order, double* ms_per_step);
 
 /* ------------------------------------------------------------------------
  * FIRE relaxation of a packing (sample preparation).

@@ -86,6 +86,11 @@ def test_batch_run_equals_single_runs():
     b = Batch(contexts(ws, skins))
     res = b.run()
     assert b.last_ms() > 0.0
+    order, ms = b.tuning()
+    print(f"block order {order}, ms per step {ms}", flush=True)
+    assert order in (-1, 0, 1, 2)
+    if order >= 0:
+        assert all(m > 0.0 for m in ms) and ms[order] == min(ms)
     for k, c in enumerate(contexts(ws, skins)):
         t = c.run()
         for f in ("peak_stress", "peak_strain", "linearity_deviation", "destruction", "step"):

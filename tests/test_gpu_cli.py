@@ -198,6 +198,8 @@ def test_cli_dcg_step_uses_the_right_runs(binary, tmp_path):
         F = W * D * D
         got_F = np.array([float(v) for v in base[3].split()[1:5]])
         np.testing.assert_allclose(got_F, F, rtol=1e-5, atol=1e-12)
+        if step == 1:
+            break  # the update after the last step is not recorded
         g = -2.0 * dR @ (W * D)
         beta = (g @ g) / (dF_old @ dF_old) if dF_old @ dF_old > 1e-12 else 0.0
         dd = g + beta * dF_old
