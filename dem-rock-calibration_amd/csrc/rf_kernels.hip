@@ -71,6 +71,11 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
     return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
 }
 
+// double4 of lane `src` of the wave.
+__device__ __forceinline__ double4 shfl_d4(const double4& v, int src) {
+    return make_double4(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src));
+}
+
 __device__ __forceinline__ int nth_bit(uint32_t m, int r) {
     for (int t = 0; t < r; ++t) m &= m - 1u;
     return m ? __builtin_ctz(m) : -1;
@@ -106,6 +111,9 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 #endif
 #ifndef RF_GATHER_BARRIER
 #define RF_GATHER_BARRIER 1
+#endif
+#ifndef RF_BOND_SPREAD
+#define RF_BOND_SPREAD 1
 #endif
 #ifndef RF_STEP_WAVES_PER_EU
 #define RF_STEP_WAVES_PER_EU 0
@@ -213,6 +221,10 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const int64_t qb0 = ic * B + l;
     const int bs0 = s.bslot[qb0];
     const int32_t bp0 = s.bpart[qb0];
+    // bond l + 8 of the particle (evaluated by a lane without a bond, below)
+    const int64_t qb1 = ic * B + (l + kLanes < B ? l + kLanes : B - 1);
+    const int bs1 = s.bslot[qb1];
+    const int32_t bp1 = s.bpart[qb1];
 
     // ---- stage 2: gathers (clamped to particle i where not needed)
     double4 pp[kPerLane];
@@ -233,13 +245,48 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         if (u == tc) jc = cj[u];
     double4 pcj = pos[jc];
     const bool pre0 = l < nbd && bs0 >= 0 && ((bnd_in >> bs0) & 1u);
-    const int32_t bj0 = pre0 ? bp0 : (int32_t)ic;
-    const double4 pb0 = pos[bj0];
-    const double4 vb0 = vel[bj0];
-    const double4 wb0 = angv[bj0];
+    const bool pre1 = l + kLanes < nbd && bs1 >= 0 && ((bnd_in >> bs1) & 1u);
+    // Bond spread: in a wave whose particles have at most 16 bonds, and no
+    // more bonds beyond the 8th than lanes without a bond of their own,
+    // bond l + 8 of lane `to`'s particle is evaluated by lane `to` in the
+    // same round as every lane's bond l (the k-th such bond by the k-th
+    // free lane), and its contributions go back to its owner lane, which
+    // adds them after bond l as it would in a second round.  Lane 0 of a
+    // group always keeps its own particle.
+    const int lane = (int)(threadIdx.x & 63u);
+    const unsigned long long msrc = __ballot(l + kLanes < nbd);
+    const unsigned long long midle = __ballot(l >= nbd && l > 0);
+    const bool spread = RF_BOND_SPREAD && msrc != 0ull &&
+                        __popcll(msrc) <= __popcll(midle) && !__any(nbd > 2 * kLanes);
+    int from = lane, to = -1;  // lane whose bond l + 8 this lane evaluates / evaluating ours
+    int32_t bjx = pre0 ? bp0 : (int32_t)ic;  // partner gathered for this lane's bond
+    int64_t qbx = pre0 ? qb0 : ic * B;      // record gathered
+    int bsx = -1;                            // slot (in its owner's table) of the spread bond
+    if (spread) {
+        unsigned long long a = msrc, f = midle;
+        while (a) {  // wave-uniform
+            const int s_ = __builtin_ctzll(a);
+            const int t_ = __builtin_ctzll(f);
+            a &= a - 1ull;
+            f &= f - 1ull;
+            if (lane == t_) from = s_;
+            if (lane == s_) to = t_;
+        }
+        const int32_t bjs = __shfl(pre1 ? bp1 : (int32_t)ic, from);
+        const int bss = __shfl(pre1 ? bs1 : -1, from);
+        if (from != lane) {
+            const int64_t io = (int64_t)blk * kPerBlock + (int64_t)((threadIdx.x & ~63u) + from) / kLanes;
+            bjx = bjs;
+            qbx = bss >= 0 ? io * B + kLanes + (from & (kLanes - 1)) : io * B;
+            bsx = bss;
+        }
+    }
+    const double4 pb0 = pos[bjx];
+    const double4 vb0 = vel[bjx];
+    const double4 wb0 = angv[bjx];
     double2 rec0[8];
     {
-        const double2* rp = reinterpret_cast<const double2*>(s.brec + (pre0 ? qb0 : ic * B));
+        const double2* rp = reinterpret_cast<const double2*>(s.brec + qbx);
 #pragma unroll
         for (int u = 0; u < 8; ++u) rec0[u] = rp[u];
     }
@@ -404,9 +451,15 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     uint32_t brk_l = 0u;
     {
         const bool live = pre0 && ((bnd >> bs0) & 1u);
+        double4 pe = pi, ve = vi, we = wi;  // the bond's own particle
+        if (spread) {
+            pe = shfl_d4(pi, from);
+            ve = shfl_d4(vi, from);
+            we = shfl_d4(wi, from);
+        }
         double c[6];
         double2 h[6];
-        const bool brk = bond_eval(p, pi, vi, wi, pb0, vb0, wb0, rec0, c, h);
+        const bool brk = bond_eval(p, pe, ve, we, pb0, vb0, wb0, rec0, c, h);
         if (live) {
 #pragma unroll
             for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c[u];
@@ -418,9 +471,29 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
                 for (int u = 0; u < 6; ++u) wp[u] = h[u];
             }
         }
+        if (spread) {
+            // a spread bond: history written by its evaluating lane, the
+            // contributions and the break flag handed back to the owner
+            const uint32_t bndo = (uint32_t)__shfl((int)bnd, from);
+            if (bsx >= 0 && ((bndo >> bsx) & 1u) && !brk) {
+                double2* wp = reinterpret_cast<double2*>(s.brec + qbx);
+#pragma unroll
+                for (int u = 0; u < 6; ++u) wp[u] = h[u];
+            }
+            const int tl = to >= 0 ? to : lane;
+            double c1[6];
+#pragma unroll
+            for (int u = 0; u < 6; ++u) c1[u] = __shfl(c[u], tl);
+            const int brk1 = __shfl((int)brk, tl);
+            if (pre1 && ((bnd >> bs1) & 1u)) {
+#pragma unroll
+                for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c1[u];
+                if (brk1) brk_l |= 1u << bs1;
+            }
+        }
     }
 #pragma unroll 1
-    for (int t = 1; t < kPerLane; ++t) {  // particles with more than 8 bonds
+    for (int t = spread ? 2 : 1; t < kPerLane; ++t) {  // particles with more than 8 bonds
         const int b = l + kLanes * t;
         if (!__any(b < nbd)) break;  // wave-uniform
         const int64_t qb = ic * B + (b < B ? b : B - 1);
