@@ -669,26 +669,24 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     }
 
     // platen partials over the block's 32 particles, tree v[t] += v[t+off]
+    // (lane t of the first wave holds v[t]; every level is one shuffle)
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double vb[kPerBlock], vt[kPerBlock];
-        int nb = 0;
+    if (threadIdx.x < 64) {
+        const int t = (int)threadIdx.x;
+        double vb = t < kPerBlock ? sh_b[t] : 0.0;
+        double vt = t < kPerBlock ? sh_t[t] : 0.0;
+        int nb = t < kPerBlock ? sh_nb[t] : 0;
 #pragma unroll
-        for (int t = 0; t < kPerBlock; ++t) {
-            vb[t] = sh_b[t];
-            vt[t] = sh_t[t];
-            nb += sh_nb[t];
+        for (int off = kPerBlock / 2; off >= 1; off >>= 1) {
+            vb = vb + __shfl_down(vb, off);
+            vt = vt + __shfl_down(vt, off);
+            nb += __shfl_down(nb, off);
         }
-#pragma unroll
-        for (int off = kPerBlock / 2; off >= 1; off >>= 1)
-#pragma unroll
-            for (int t = 0; t < off; ++t) {
-                vb[t] = vb[t] + vb[t + off];
-                vt[t] = vt[t] + vt[t + off];
-            }
-        s.part_b[blk] = vb[0];
-        s.part_t[blk] = vt[0];
-        s.part_bonds[blk] = nb;
+        if (t == 0) {
+            s.part_b[blk] = vb;
+            s.part_t[blk] = vt;
+            s.part_bonds[blk] = nb;
+        }
     }
 }
 
@@ -719,8 +717,9 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     __shared__ double sb[kCtrlThreads];
     __shared__ double stt[kCtrlThreads];
     __shared__ long long sn[kCtrlThreads];
-    if (ctrl->halt) return;
     const int t = threadIdx.x;
+    // the partials are read before the halt flag is known (a halted step
+    // leaves them as they were), so both loads are in flight together
     double ab = 0.0, at = 0.0;
     long long nb = 0;
 #pragma unroll 4
@@ -729,11 +728,14 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
         at = at + s.part_t[b];
         nb += s.part_bonds[b];
     }
+    if (ctrl->halt) return;
+    // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
+    // one shuffle per level
     sb[t] = ab;
     stt[t] = at;
     sn[t] = nb;
     __syncthreads();
-    for (int off = kCtrlThreads / 2; off >= 1; off >>= 1) {
+    for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
         if (t < off) {
             sb[t] = sb[t] + sb[t + off];
             stt[t] = stt[t] + stt[t + off];
@@ -741,7 +743,19 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
         }
         __syncthreads();
     }
+    if (t >= 64) return;
+    double vb = sb[t], vt = stt[t];
+    long long vn = sn[t];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        vb = vb + __shfl_down(vb, off);
+        vt = vt + __shfl_down(vt, off);
+        vn += __shfl_down(vn, off);
+    }
     if (t != 0) return;
+    sb[0] = vb;
+    stt[0] = vt;
+    sn[0] = vn;
     Ctrl c = *ctrl;
     if (p.kind != RF_TEST_NONE) {
         c.sfl0 = sb[0];
