@@ -96,27 +96,68 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
         const Vec4<D> wi = load4(angv[i], t.tang[cur], i, dir);
         const D ri = pi.w, mi = vi.w;
         uint32_t occ = s.occ[i], bnd = s.bnd[i], otnz = s.otnz[i];
-        if (lead) {
-            // The slot table is held fixed.  Pairs that cross acut (> r_i + r_j)
-            // carry no force, so leaving them in or out changes nothing; the
-            // plain step catches up on them afterwards.  A pair that touches
-            // without a slot, or coincides, ends the window.
-            for (uint32_t q = occ; q; q &= q - 1u) {
-                const int k = __builtin_ctz(q);
-                const double4 pj = pos[s.nbr[i * S + k]];
-                const double dx = pj.x - pi.x.v, dy = pj.y - pi.y.v, dz = pj.z - pi.z.v;
-                const double rr = dx * dx + dy * dy + dz * dz;
-                if (rr < 1e-10) atomicOr(&ctrl->err, kErrSensList);
+        // The slot table is held fixed.  Pairs that cross acut (> r_i + r_j)
+        // carry no force, so leaving them in or out changes nothing; the
+        // plain step catches up on them afterwards.  A pair that touches
+        // without a slot, or coincides, ends the window.  The particle's
+        // lanes share the checks (blocks of 8 slots or candidates, every
+        // gather of a block issued at once) and also find which slots
+        // overlap, for the contact sums below.
+        const int base = lane - dir;  // the particle's first lane
+        uint32_t ovl_l = 0u;          // overlapping occupied slots (this lane's blocks)
+        bool bad = false;
+        for (int j = dir; j < kMaxSlots / 8; j += nl) {
+            const uint32_t blk = (occ >> (8 * j)) & 0xffu;
+            if (!blk) continue;
+            int32_t jn[8];
+            double4 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = 8 * j + u;
+                jn[u] = s.nbr[i * S + (k < S ? k : S - 1)];
             }
-            for (int c = 0; c < s.ncand[i] && c < C; ++c) {
-                if (s.cslot[i * C + c] >= 0) continue;
-                const double4 pj = pos[s.cand[i * C + c]];
-                const double dx = pj.x - pi.x.v, dy = pj.y - pi.y.v, dz = pj.z - pi.z.v;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = pos[((blk >> u) & 1u) ? jn[u] : (int32_t)i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (!((blk >> u) & 1u)) continue;
+                const double dx = q[u].x - pi.x.v, dy = q[u].y - pi.y.v, dz = q[u].z - pi.z.v;
                 const double rr = dx * dx + dy * dy + dz * dz;
-                const double rs = pi.w.v + pj.w;
-                if (rr < rs * rs) atomicOr(&ctrl->err, kErrSensList);
+                bad = bad || rr < 1e-10;
+                if (::sqrt(rr) < pi.w.v + q[u].w) ovl_l |= 1u << (8 * j + u);
             }
         }
+        const int ncd = s.ncand[i] < C ? s.ncand[i] : C;
+        for (int j = dir; 8 * j < ncd; j += nl) {
+            int32_t jc[8];
+            int cs[8];
+            double4 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = 8 * j + u;
+                const int64_t qc = i * C + (c < C ? c : C - 1);
+                jc[u] = s.cand[qc];
+                cs[u] = s.cslot[qc];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                q[u] = pos[(8 * j + u < ncd && cs[u] < 0) ? jc[u] : (int32_t)i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (!(8 * j + u < ncd && cs[u] < 0)) continue;
+                const double dx = q[u].x - pi.x.v, dy = q[u].y - pi.y.v, dz = q[u].z - pi.z.v;
+                const double rr = dx * dx + dy * dy + dz * dz;
+                const double rs = pi.w.v + q[u].w;
+                bad = bad || rr < rs * rs;
+            }
+        }
+        uint32_t ovl_all = 0u;
+        int bad_all = 0;
+        for (int d = 0; d < nl; ++d) {
+            ovl_all |= (uint32_t)__shfl((int)ovl_l, base + d);
+            bad_all |= __shfl((int)bad, base + d);
+        }
+        if (bad_all && lead) atomicOr(&ctrl->err, kErrSensList);
         const D zero = dconst<1>(0.0);
         D F[3] = {zero, zero, zero};
         D M[3] = {zero, zero, zero};
@@ -149,14 +190,31 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 }
             }
         };
-        // bonds in creation order; entry b goes to lane b % 8
+        // bonds in creation order; entry b goes to lane b % 8.  The slots
+        // and partners of the first 16 bonds are loaded up front (rows of
+        // 32 entries: 16-byte aligned).
         const int nbd = s.nbond[i];
+        const uint4 bsl = *reinterpret_cast<const uint4*>(s.bslot + i * B);
+        int4 bpr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bpr[q] = reinterpret_cast<const int4*>(s.bpart + i * B)[q];
         auto bond_lane = [&](int l, D (&acc)[6]) {
 #pragma unroll 1
             for (int b = l; b < nbd && b < B; b += kLanes) {
-                const int ks = s.bslot[i * B + b];
+                int ks;
+                int64_t j;
+                if (b < 16) {
+                    const uint32_t w = (b >> 2) == 0 ? bsl.x : (b >> 2) == 1 ? bsl.y
+                                                           : (b >> 2) == 2 ? bsl.z : bsl.w;
+                    ks = (int)(int8_t)(uint8_t)(w >> (8 * (b & 3)));
+                    const int4 e = (b >> 2) == 0 ? bpr[0] : (b >> 2) == 1 ? bpr[1]
+                                                          : (b >> 2) == 2 ? bpr[2] : bpr[3];
+                    j = (b & 3) == 0 ? e.x : (b & 3) == 1 ? e.y : (b & 3) == 2 ? e.z : e.w;
+                } else {
+                    ks = s.bslot[i * B + b];
+                    j = s.bpart[i * B + b];
+                }
                 if (ks < 0 || !((bnd >> ks) & 1u)) continue;
-                const int64_t j = s.bpart[i * B + b];
                 const int64_t qb = i * B + b;
                 const Vec4<D> pj = load4(pos[j], t.tpos[cur], j, dir);
                 const Vec4<D> vj = load4(vel[j], t.tvel[cur], j, dir);
@@ -194,17 +252,14 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
         const D wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
         const bool roll = wim.v > 1e-18;
         const uint32_t cmask = occ & ~bnd;
-        uint32_t ovl = 0u;
+        const uint32_t ovl = cmask & ovl_all;  // contacts that touch
         auto contact_lane = [&](int l, D (&acc)[6]) {
 #pragma unroll 1
             for (int k = l; k < 32; k += kLanes) {
-                if (!((cmask >> k) & 1u)) continue;
+                if (!((ovl >> k) & 1u)) continue;
                 const int64_t j = s.nbr[i * S + k];
                 const int64_t qs = i * S + k;
                 const double4 pjv = pos[j];
-                const double dx = pjv.x - pi.x.v, dy = pjv.y - pi.y.v, dz = pjv.z - pi.z.v;
-                if (!(::sqrt(dx * dx + dy * dy + dz * dz) < pi.w.v + pjv.w)) continue;
-                ovl |= 1u << k;
                 const Vec4<D> pj = load4(pjv, t.tpos[cur], j, dir);
                 const Vec4<D> vj = load4(vel[j], t.tvel[cur], j, dir);
                 const Vec4<D> wj = load4(angv[j], t.tang[cur], j, dir);
@@ -395,6 +450,7 @@ __global__ __launch_bounds__(256) void k_sens_ctrl(DevParams p, SensState t, Ctr
 
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
                             hipStream_t st) {
+    if (p.B != kMaxBonds) return hipErrorInvalidValue;  // bond rows read 16 entries at once
     const int64_t per_block = (int64_t)(kSensBlock / 64) * (64 / t.nl);
     hipLaunchKernelGGL(k_sens_step, dim3((unsigned)((p.n + per_block - 1) / per_block)),
                        dim3(kSensBlock), 0, st, s, p, t, ctrl);
