@@ -999,16 +999,19 @@ __global__ void k_prune_initial(DevState s, DevParams p, Ctrl* ctrl) {
     s.otnz[i] &= ~drop;
 }
 
-// Candidate lists: every j within acut + skin (27 cells of edge 2 acut) that
+// Candidate lists: every j within acut + skin (cells of edge >= 2 acut) that
 // is not a current row partner.  Particles at the hidden point neither have
 // nor are candidates: they sit at one point outside the box (rho = 0 never
 // links, d_ReConstructInteractionListbpm's 1e-10 < rr) and never move again.
-// Eight lanes per particle, lane l scanning stencil cells l, l+8, l+16, l+24:
-// a counting pass, a prefix over the eight counts, and a writing pass, so the
-// list order is fixed (lane by lane, cell by cell, sorted order in a cell).
+// Eight lanes per particle share the cells that can reach acut + skin; each
+// lane keeps its hits in LDS and a prefix over the eight counts places them,
+// so the list order is fixed (lane by lane, cell by cell, sorted order in a
+// cell).  The current partners are held in registers for the exclusion.
 // The order of a list never changes a result: new pairs are ranked by
 // stencil cell and id when they are appended.
 __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams p, Ctrl* ctrl) {
+    // each lane's first kMaxCand hits, entry k of thread t at [k * 256 + t]
+    __shared__ int32_t hitbuf[kMaxCand * 256];
     const int l = threadIdx.x & (kLanes - 1);
     const int64_t i = (int64_t)blockIdx.x * (256 / kLanes) + (threadIdx.x >> 3);
     const bool valid = i < p.n;
@@ -1020,6 +1023,26 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t C = p.C;
     const uint32_t occ = s.occ[ic];
+    // the particle's current partners (-1 where a slot is free), held by
+    // every lane of the group
+    int32_t row[kMaxSlots];
+    if (p.S == kMaxSlots) {  // 128-byte rows: eight 16-byte loads
+        const int4* rp = reinterpret_cast<const int4*>(s.nbr + ic * kMaxSlots);
+#pragma unroll
+        for (int q = 0; q < kMaxSlots / 4; ++q) {
+            const int4 v = rp[q];
+            row[4 * q] = v.x;
+            row[4 * q + 1] = v.y;
+            row[4 * q + 2] = v.z;
+            row[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kMaxSlots; ++k) row[k] = s.nbr[ic * p.S + (k < p.S ? k : p.S - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k)
+        if (!((occ >> k) & 1u)) row[k] = -1;
     // bit u of the result: entry q + u of the cell is a candidate (four
     // entries per round, their loads issued together)
     auto hits4 = [&](int32_t st, int32_t ct, int32_t q, int32_t (&j)[4]) -> uint32_t {
@@ -1039,8 +1062,8 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
         for (uint32_t b = m4; b; b &= b - 1u) {  // current partners hold a slot already
             const int u = __builtin_ctz(b);
             bool partner = false;
-            for (uint32_t m = occ; m && !partner; m &= m - 1u)
-                partner = s.nbr[ic * p.S + __builtin_ctz(m)] == j[u];
+#pragma unroll
+            for (int k = 0; k < kMaxSlots; ++k) partner |= row[k] == j[u];
             if (partner) m4 &= ~(1u << u);
         }
         return m4;
@@ -1048,13 +1071,39 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     auto cell_of = [&](int c) {
         return key + (c % 3 - 1) + ((c / 3) % 3 - 1) * nx + (c / 9 - 1) * nxy;
     };
+    // Cells that can hold a point within R = acut + skin (R <= cell edge):
+    // along each axis the own cell, the lower neighbour when the particle
+    // is closer than R to the cell's lower face, the upper one likewise
+    // (with cell edge 2 acut and skin 0.08 acut, about 9 of the 27).
+    const double R = sqrt(p.cand_r2) * (1.0 + 1e-9);
+    const double fx = floor(pi.x * p.inv_cell), fy = floor(pi.y * p.inv_cell),
+                 fz = floor(pi.z * p.inv_cell);
+    const double ox = pi.x - fx * p.cell, oy = pi.y - fy * p.cell, oz = pi.z - fz * p.cell;
+    const uint32_t ax = 2u | (uint32_t)(ox < R) | ((uint32_t)(p.cell - ox < R) << 2);
+    const uint32_t ay = 2u | (uint32_t)(oy < R) | ((uint32_t)(p.cell - oy < R) << 2);
+    const uint32_t az = 2u | (uint32_t)(oz < R) | ((uint32_t)(p.cell - oz < R) << 2);
+    uint32_t need = 0u;  // bit c: stencil cell c (offsets c%3-1, (c/3)%3-1, c/9-1)
+#pragma unroll
+    for (int c = 0; c < 27; ++c)
+        if (((ax >> (c % 3)) & (ay >> ((c / 3) % 3)) & (az >> (c / 9))) & 1u) need |= 1u << c;
+    // one pass: hits counted and the first kMaxCand kept in LDS; the k-th
+    // needed cell goes to lane k % 8, and the list is lane-major (each
+    // lane's cells in order), as positions come from the lanes' prefix sum
     int cnt = 0;
-    for (int c = l; c < 27 && !skip; c += kLanes) {
+    uint32_t m = skip ? 0u : need;
+    for (int t = 0; t < l; ++t) m &= m - 1u;  // this lane starts at the l-th needed cell
+    while (m) {
+        const int c = __builtin_ctz(m);
+#pragma unroll
+        for (int t = 0; t < kLanes; ++t) m &= m - 1u;  // then every 8th
         const int64_t cc = cell_of(c);
         const int32_t st = s.cstart[cc], ct = s.ccount[cc];
         for (int32_t q = 0; q < ct; q += 4) {
             int32_t j[4];
-            cnt += __popc(hits4(st, ct, q, j));
+            for (uint32_t b = hits4(st, ct, q, j); b; b &= b - 1u) {
+                if (cnt < kMaxCand) hitbuf[cnt * 256 + threadIdx.x] = j[__builtin_ctz(b)];
+                ++cnt;
+            }
         }
     }
     int incl = cnt;
@@ -1064,19 +1113,10 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
         if (l >= off) incl += t;
     }
     const int total = __shfl(incl, kLanes - 1, kLanes);
-    int w = incl - cnt;
-    for (int c = l; c < 27 && !skip && w < C; c += kLanes) {
-        const int64_t cc = cell_of(c);
-        const int32_t st = s.cstart[cc], ct = s.ccount[cc];
-        for (int32_t q = 0; q < ct && w < C; q += 4) {
-            int32_t j[4];
-            for (uint32_t b = hits4(st, ct, q, j); b && w < C; b &= b - 1u) {
-                const int u = __builtin_ctz(b);
-                s.cand[ic * C + w] = j[u];
-                s.cslot[ic * C + w] = -1;
-                ++w;
-            }
-        }
+    const int w0 = incl - cnt;
+    for (int k = 0; k < cnt && k < kMaxCand && w0 + k < C; ++k) {
+        s.cand[ic * C + w0 + k] = hitbuf[k * 256 + threadIdx.x];
+        s.cslot[ic * C + w0 + k] = -1;
     }
     if (valid && l == 0) {
         if (total > C) atomicOr(&ctrl->err, kErrCandOverflow);
