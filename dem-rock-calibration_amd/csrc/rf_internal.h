@@ -136,6 +136,8 @@ struct SensState {
 
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
                             hipStream_t st);
+hipError_t launch_controller_sens(const DevState& s, const DevParams& p, const SensState& t,
+                                  Ctrl* ctrl, hipStream_t st);
 
 // One sample of a batched launch: its state, constants and control block.
 struct BatchEntry {

@@ -822,6 +822,59 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevPara
     ctrl_body(s, p, ctrl);
 }
 
+// Tangents of the platen forces where the value controller records a curve
+// point: per direction the block partials summed strided over the block's
+// threads and by the tree through LDS and shuffles, stored beside the point.
+// Reads the step and curve index before the value controller advances them.
+__device__ __forceinline__ void sens_ctrl_body(const DevParams& p, const SensState& t,
+                                               Ctrl* __restrict__ ctrl) {
+    __shared__ double tb[kCtrlThreads], tt[kCtrlThreads];
+    if (ctrl->halt || p.kind == RF_TEST_NONE) return;
+    const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
+    const int64_t step = ctrl->step, save = ctrl->stepsave;
+    if (step % dn != 0 || save >= p.curve_cap) return;  // uniform
+    const int tid = threadIdx.x;
+    for (int k = 0; k < t.nl; ++k) {
+        double ab = 0.0, at = 0.0;
+        for (int b = tid; b < p.nblk; b += kCtrlThreads) {
+            ab = ab + t.tpart_b[k * p.nblk + b];
+            at = at + t.tpart_t[k * p.nblk + b];
+        }
+        tb[tid] = ab;
+        tt[tid] = at;
+        __syncthreads();
+        for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
+            if (tid < off) {
+                tb[tid] = tb[tid] + tb[tid + off];
+                tt[tid] = tt[tid] + tt[tid + off];
+            }
+            __syncthreads();
+        }
+        if (tid < 64) {
+            double vb = tb[tid], vt = tt[tid];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                vb = vb + __shfl_down(vb, off);
+                vt = vt + __shfl_down(vt, off);
+            }
+            if (tid == 0) {
+                t.tcurve[(save * kSensK + k) * 2] = vb * p.inv_area;
+                t.tcurve[(save * kSensK + k) * 2 + 1] = vt * p.inv_area;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Controller of a sensitivity step: the tangents of a curve point, then the
+// value controller.
+__global__ __launch_bounds__(kCtrlThreads) void k_controller_sens(DevState s, DevParams p,
+                                                                  SensState t,
+                                                                  Ctrl* __restrict__ ctrl) {
+    sens_ctrl_body(p, t, ctrl);
+    ctrl_body(s, p, ctrl);
+}
+
 // One block per sample.
 __global__ __launch_bounds__(kCtrlThreads) void k_controller_batch(const BatchEntry* __restrict__ e) {
     const BatchEntry& b = e[blockIdx.x];
@@ -1251,6 +1304,12 @@ hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample,
                              const int32_t* blk_local, int32_t nblk_total, hipStream_t st) {
     hipLaunchKernelGGL(k_step_batch, dim3((unsigned)nblk_total), dim3(kBlock), 0, st, e,
                        blk_sample, blk_local);
+    return hipGetLastError();
+}
+
+hipError_t launch_controller_sens(const DevState& s, const DevParams& p, const SensState& t,
+                                  Ctrl* ctrl, hipStream_t st) {
+    hipLaunchKernelGGL(k_controller_sens, dim3(1), dim3(kCtrlThreads), 0, st, s, p, t, ctrl);
     return hipGetLastError();
 }
 

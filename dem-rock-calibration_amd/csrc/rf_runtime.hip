@@ -1569,7 +1569,7 @@ rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         const int64_t before = c->hctrl.step;
         for (int64_t s = 0; s < k; ++s) {
             HIP_TRY(rf::launch_sens_step(c->d, c->p, c->sens, c->ctrl, c->stream));
-            HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+            HIP_TRY(rf::launch_controller_sens(c->d, c->p, c->sens, c->ctrl, c->stream));
         }
         st = sync_ctrl(c);
         if (st) return st;

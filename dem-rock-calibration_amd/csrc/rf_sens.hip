@@ -13,8 +13,10 @@
 // of a particle writes the values, every lane its tangent slice; tangent
 // arrays are laid out [entity][kSensK] so the lanes of a particle read and
 // write one contiguous run.  Platen forces go out per particle and a second
-// kernel adds them per 32 particles in the value controller's tree order.  Per-particle sums follow the step kernel's lane order, so
-// the values are the step kernel's bit for bit.  The slot table is held
+// kernel adds them per 32 particles in the value controller's tree order;
+// the controller of a sensitivity step (k_controller_sens) adds the
+// tangents where a curve point is recorded.  Per-particle sums follow the
+// step kernel's lane order, so the values are the step kernel's bit for bit.  The slot table is held
 // fixed: pairs crossing acut carry no force either way, and a touching pair
 // without a slot ends the window with an error.  The step size dt is a
 // constant of the run.
@@ -411,43 +413,6 @@ __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t,
     }
 }
 
-// Tangents of the platen forces; records the curve's tangents where the
-// value controller (launched next) records a curve point.
-__global__ __launch_bounds__(256) void k_sens_ctrl(DevParams p, SensState t, Ctrl* __restrict__ ctrl) {
-    __shared__ double sb[kSensK][256], st[kSensK][256];
-    if (ctrl->halt) return;
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < kSensK; ++k) {
-        double ab = 0.0, at = 0.0;
-        for (int b = tid; b < p.nblk; b += 256) {
-            ab = ab + t.tpart_b[k * p.nblk + b];
-            at = at + t.tpart_t[k * p.nblk + b];
-        }
-        sb[k][tid] = ab;
-        st[k][tid] = at;
-    }
-    __syncthreads();
-    for (int off = 128; off >= 1; off >>= 1) {
-        if (tid < off)
-#pragma unroll
-            for (int k = 0; k < kSensK; ++k) {
-                sb[k][tid] += sb[k][tid + off];
-                st[k][tid] += st[k][tid + off];
-            }
-        __syncthreads();
-    }
-    if (tid != 0 || p.kind == RF_TEST_NONE) return;
-    const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
-    const int64_t step = ctrl->step, save = ctrl->stepsave;
-    if (step % dn == 0 && save < p.curve_cap)
-#pragma unroll
-        for (int k = 0; k < kSensK; ++k) {
-            t.tcurve[(save * kSensK + k) * 2] = sb[k][0] * p.inv_area;
-            t.tcurve[(save * kSensK + k) * 2 + 1] = st[k][0] * p.inv_area;
-        }
-}
-
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
                             hipStream_t st) {
     if (p.B != kMaxBonds) return hipErrorInvalidValue;  // bond rows read 16 entries at once
@@ -456,7 +421,6 @@ hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensSta
                        dim3(kSensBlock), 0, st, s, p, t, ctrl);
     hipLaunchKernelGGL(k_sens_partials, dim3((unsigned)((p.nblk + 7) / 8)), dim3(256), 0, st, p, t,
                        s, ctrl);
-    hipLaunchKernelGGL(k_sens_ctrl, dim3(1), dim3(256), 0, st, p, t, ctrl);
     return hipGetLastError();
 }
 
