@@ -59,6 +59,35 @@ def test_uniaxial_steps_parity(small_uniaxial):
     np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
 
 
+@pytest.mark.parametrize("gap_factor", [0.3, 1.0])
+def test_dense_bonds_parity(gap_factor):
+    """Bond gaps of 0.3 and 1.0 r_min: 37 % / 83 % of the particles hold more
+    than 8 bonds (up to 14 / 19), so the step kernel's waves take every bond
+    path: bonds 9-16 evaluated by lanes without a bond of their own, the
+    second round where a wave has more such bonds than free lanes, and a
+    third round past 16 bonds."""
+    print(f"case dense bonds gap_factor={gap_factor}", flush=True)
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="uniaxial", seed=31,
+                             mat_kwargs=dict(gap_factor=gap_factor),
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=5, max_strain=0.05))
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    nb = (np.asarray(o.get_state()["type"]).reshape(o.n, -1) == 1).sum(axis=1)
+    assert (nb > 8).mean() > 0.3
+    if gap_factor >= 1.0:
+        assert nb.max() > 16
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    done = 0
+    for chunk in (1, 49, 150):
+        assert g.step(chunk) == o.step(chunk)
+        done += chunk
+        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"step {done}")
+        compare_test_states(g.test_state(), o.test_state(), what=f"step {done}")
+    np.testing.assert_array_equal(g.curve(), o.curve())
+
+
 def test_free_motion_append_parity():
     """Random velocities, no platens: contacts open and close, rows are
     appended and pruned every step; small skin forces frequent rebuilds."""
