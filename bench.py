@@ -167,7 +167,8 @@ def batch_bench(count, steps, warmup):
                        "kind": "same samples one after another on the GPU"},
         "speedup": pst_bat / pst_seq,
         "sample_steps_per_s": {"batched": bat_steps / t_bat, "sequential": seq_steps / t_seq},
-        "block_order": {"chosen": ["by sample", "interleaved", "largest first"][order]
+        "block_order": {"chosen": ["by sample", "interleaved", "largest first",
+                                   "one after another"][order]
                         if order >= 0 else None, "ms_per_step": tune_ms},
     }))
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1115,14 +1116,19 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
 //
 // Autotuning: the order of the block table is free (a block's work depends
 // only on its sample and its local index).  The first steps of a batch run
-// short chunks under each candidate order, timed with events on the batch
-// stream, and the fastest order is kept: samples one after another, blocks
-// of all samples interleaved, or samples one after another largest first.
-// Full chunks replay as captured graphs (their kernels read the tables and
-// entries from device memory, so one graph per chunk length serves every
-// order), and chunks shorten while list rebuilds come often.
+// short chunks in each of four ways, timed on the host clock around the
+// synchronised chunk, and the fastest is kept: one launch per step with the
+// samples' blocks one sample after another, interleaved, or largest sample
+// first, or no shared launch at all -- each sample stepped on its own
+// stream, one after another, as rf_step does (large samples fill the GPU
+// alone, and the shared launch's per-block table lookups then cost more
+// than they save).  Full chunks replay as captured graphs (their kernels
+// read the tables and entries from device memory, so one graph per chunk
+// length serves every order), and chunks shorten while list rebuilds come
+// often.
 
-constexpr int kBatchOrders = 3;
+constexpr int kBatchOrders = 4;       // three block orders and one after another
+constexpr int kBatchSequential = 3;
 constexpr int kTuneSteps = 32;
 
 struct rf_batch {
@@ -1138,7 +1144,7 @@ struct rf_batch {
     int32_t order = -1;      // table order in use
     int32_t chosen = -1;     // order picked by the tuner (-1: not yet)
     int32_t tuning = 0;      // next order to time
-    double tune_ms[kBatchOrders] = {0.0, 0.0, 0.0};
+    double tune_ms[kBatchOrders] = {0.0, 0.0, 0.0, 0.0};
     double last_ms = 0.0;
     struct Graph {
         int64_t steps = 0;
@@ -1322,60 +1328,82 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
     if (st) return st;
     st = batch_sync(b);
     if (st) return st;
-    hipEvent_t t0 = nullptr, t1 = nullptr, ta = nullptr, tb = nullptr;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
-    HIP_TRY(hipEventCreate(&ta));
-    HIP_TRY(hipEventCreate(&tb));
-    HIP_TRY(hipEventRecord(t0, b->stream));
+    std::vector<int64_t> stop(count);
+    for (size_t q = 0; q < count; ++q)
+        stop[q] = nsteps > INT64_MAX / 4 ? INT64_MAX / 4 : start[q] + nsteps;
+    using clock = std::chrono::steady_clock;
+    const auto t_begin = clock::now();
     rf_status result = RF_OK;
     for (;;) {
         bool active = false;
         for (rf_ctx* c : b->c) active = active || !c->hctrl.halt;
         if (!active) break;
         const bool tune = b->chosen < 0;
-        if (tune) {
+        const int mode = tune ? b->tuning : b->chosen;
+        const bool seq = mode == kBatchSequential;
+        if (tune && !seq) {
             st = batch_order(b, b->tuning);
             if (st) { result = st; break; }
         }
         const int64_t k = tune ? kTuneSteps : b->chunk_eff;
-        std::vector<int64_t> before(count);
+        std::vector<int64_t> before(count), reb(count);
         std::vector<char> idle(count);  // halted for good before the chunk
         for (size_t q = 0; q < count; ++q) {
             before[q] = b->c[q]->hctrl.step;
+            reb[q] = b->c[q]->rebuilds;
             idle[q] = b->c[q]->hctrl.halt != 0;
         }
-        HIP_TRY(hipEventRecord(ta, b->stream));
-        st = batch_launch(b, k, !tune && k > 1);
-        if (st) { result = st; break; }
-        HIP_TRY(hipEventRecord(tb, b->stream));
-        st = batch_sync(b);
-        if (st) { result = st; break; }
+        const auto ta = clock::now();
         bool progress = false, rebuilt = false, full = true;
-        for (size_t q = 0; q < count && !result; ++q) {
-            rf_ctx* c = b->c[q];
-            progress = progress || c->hctrl.step != before[q];
-            full = full && (idle[q] || c->hctrl.step - before[q] == k);
-            if (c->hctrl.err) { result = device_error(c); break; }
-            if (c->hctrl.rebuild) {
-                st = rebuild(c);
-                if (st) { result = st; break; }
-                if (c->hctrl.err) { result = device_error(c); break; }
-                rebuilt = true;
+        if (seq) {
+            // every sample on its own stream, one after another (rf_step
+            // handles its rebuilds and keeps its control block in sync)
+            for (size_t q = 0; q < count && !result; ++q) {
+                rf_ctx* c = b->c[q];
+                if (idle[q] || c->hctrl.done) continue;
+                const int64_t left = stop[q] - c->hctrl.step;
+                if (left <= 0) continue;
+                st = rf_step(c, std::min<int64_t>(k, left), nullptr);
+                if (st) result = st;
             }
+            if (result) break;
+            for (size_t q = 0; q < count; ++q) {
+                rf_ctx* c = b->c[q];
+                progress = progress || c->hctrl.step != before[q];
+                full = full && (idle[q] || c->hctrl.step - before[q] == k);
+                rebuilt = rebuilt || c->rebuilds != reb[q];
+            }
+        } else {
+            st = batch_launch(b, k, !tune && k > 1);
+            if (st) { result = st; break; }
+            st = batch_sync(b);
+            if (st) { result = st; break; }
+            for (size_t q = 0; q < count && !result; ++q) {
+                rf_ctx* c = b->c[q];
+                progress = progress || c->hctrl.step != before[q];
+                full = full && (idle[q] || c->hctrl.step - before[q] == k);
+                if (c->hctrl.err) { result = device_error(c); break; }
+                if (c->hctrl.rebuild) {
+                    st = rebuild(c);
+                    if (st) { result = st; break; }
+                    if (c->hctrl.err) { result = device_error(c); break; }
+                    rebuilt = true;
+                }
+            }
+            if (result) break;
         }
-        if (result) break;
-        if (tune && full) {  // a clean timing: every active sample ran every step
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, ta, tb));
-            b->tune_ms[b->tuning] = ms / (double)k;
+        const double chunk_ms = std::chrono::duration<double, std::milli>(clock::now() - ta).count();
+        if (tune && full && !rebuilt) {  // a clean timing: every active sample ran every step
+            b->tune_ms[b->tuning] = chunk_ms / (double)k;
             if (++b->tuning == kBatchOrders) {
                 int best = 0;
                 for (int o = 1; o < kBatchOrders; ++o)
                     if (b->tune_ms[o] < b->tune_ms[best]) best = o;
                 b->chosen = best;
-                st = batch_order(b, best);
-                if (st) { result = st; break; }
+                if (best != kBatchSequential) {
+                    st = batch_order(b, best);
+                    if (st) { result = st; break; }
+                }
             }
         } else if (!tune) {
             if (rebuilt && !full) {
@@ -1396,15 +1424,7 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
             break;
         }
     }
-    hipEventDestroy(ta);
-    hipEventDestroy(tb);
-    HIP_TRY(hipEventRecord(t1, b->stream));
-    HIP_TRY(hipEventSynchronize(t1));
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, t0, t1);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    b->last_ms = ms;
+    b->last_ms = std::chrono::duration<double, std::milli>(clock::now() - t_begin).count();
     for (size_t k = 0; k < count; ++k) HIP_TRY(rf::launch_set_stop(b->c[k]->ctrl, 0, b->stream));
     st = batch_sync(b);
     if (st && !result) result = st;

@@ -246,8 +246,10 @@ class Batch:
         return ms.value
 
     def tuning(self):
-        """(chosen block order or -1, [ms per step of each order])."""
+        """(choice or -1, [ms per step of each way]): 0-2 one shared launch
+        with the blocks by sample, interleaved, largest sample first; 3 each
+        sample alone, one after another."""
         order = C.c_int32()
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * 4)()
         check(lib().rf_batch_tuning(self.h, C.byref(order), ms), "rf_batch_tuning")
         return order.value, list(ms)

@@ -327,14 +327,15 @@ rf_status rf_batch_destroy(rf_batch* batch);
 rf_status rf_batch_step(rf_batch* batch, int64_t nsteps, int64_t* done_steps);
 /* Run every sample's platen test to its end; out has count entries. */
 rf_status rf_batch_run(rf_batch* batch, rf_test_state* out);
-/* Wall time (ms) of the last rf_batch_step / rf_batch_run on the device. */
+/* Wall time (ms) of the last rf_batch_step / rf_batch_run. */
 rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
 /* The batch's launch autotuner (no reference counterpart): the first steps of
- * a batch run in short chunks under each of three block orders (samples one
- * after another, blocks of all samples interleaved, samples largest first),
- * timed on the device, and the fastest order is kept.  The order never
- * changes a result.  *order receives the chosen order (-1 before tuning),
- * ms_per_step (3 entries, may be null) the measured step times. */
+ * a batch run in short chunks in each of four ways -- one shared launch per
+ * step with the blocks ordered by sample (0), interleaved (1) or largest
+ * sample first (2), or each sample stepped alone, one after another (3) --
+ * timed on the host, and the fastest is kept.  The choice never changes a
+ * result.  *order receives the choice (-1 before tuning), ms_per_step
+ * (4 entries, may be null) the measured step times. */
 rf_status rf_batch_tuning(rf_batch* batch, int32_t* order, double* ms_per_step);
 
 /* ------------------------------------------------------------------------
