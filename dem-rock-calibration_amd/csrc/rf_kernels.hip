@@ -497,8 +497,9 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const int b = l + kLanes * t;
         if (!__any(b < nbd)) break;  // wave-uniform
         const int64_t qb = ic * B + (b < B ? b : B - 1);
-        const int bks = b < nbd && b < B ? (int)s.bslot[qb] : -1;
-        const int32_t bjr = s.bpart[qb];
+        // bond l + 8's slot and partner came with the first loads
+        const int bks = !(b < nbd && b < B) ? -1 : t == 1 ? bs1 : (int)s.bslot[qb];
+        const int32_t bjr = t == 1 ? bp1 : s.bpart[qb];
         const bool live = bks >= 0 && ((bnd >> bks) & 1u);
         const int32_t bj = live ? bjr : (int32_t)ic;
         const double4 pj = pos[bj];
