@@ -86,7 +86,7 @@ def cpu_baseline(w, steps, threads):
     k = o.step(steps)
     dt = time.perf_counter() - t0
     return dict(value=w["n"] * k / dt, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{k} loading steps of the same {w['n']}-particle uniaxial workload "
+                sample=f"{k} loading steps of the same {w['n']}-particle workload "
                        f"after 2 warm-up steps, FP64 CPU oracle, OpenMP {threads} threads")
 
 
@@ -264,7 +264,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=2000)
-    ap.add_argument("--n", type=int, default=100000)
+    ap.add_argument("--n", type=int, default=0,
+                    help="particles (default: the workload's own count)")
+    ap.add_argument("--workload", default="uniaxial-100k",
+                    choices=["uniaxial-100k", "uniaxial-5k", "brazilian-50k"],
+                    help="uniaxial-100k: 2:1 cylinder, r ~ U[0.20, 0.35] mm, strain rate "
+                         "0.1/s (the default line); uniaxial-5k: the same test with "
+                         "r ~ U[0.20, 0.30] mm; brazilian-50k: 2:1 disc, r ~ U[0.20, 0.30] mm, "
+                         "platens at 0.02 m/s")
     ap.add_argument("--cpu-steps", type=int, default=-1,
                     help="CPU-baseline steps (default: sized to ~15 s of CPU work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -302,13 +309,28 @@ def main():
     from rockfit.core import Context
 
     t0 = time.perf_counter()
-    w = config.make_workload(n=args.n, r_min=0.20, r_max=0.35, test="uniaxial")
-    # strain rate 0.1/s: closing speed = 0.1/s * specimen height
-    height_mm = w["cfg"].size[2]
-    v_mps = -0.1 * height_mm * 1e-3
-    w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v_mps, dnsave=100)
+    wl = args.workload
+    n_default = {"uniaxial-100k": 100000, "uniaxial-5k": 5000, "brazilian-50k": 50000}[wl]
+    n = args.n or n_default
+    if wl == "brazilian-50k":
+        w = config.make_workload(n=n, r_min=0.20, r_max=0.30, shape="disc", aspect=0.5,
+                                 test="brazilian")
+        v_mps = -0.02
+        w["cfg"] = config.brazilian_cfg(w["xyz"], w["r"], v_load_mps=v_mps, dnsave=100)
+        desc = (f"BPM Brazilian disc test, {n} spheres r~U[0.20,0.30] mm, 2:1 disc, "
+                f"platens at 0.02 m/s, FP64")
+        metric = f"particle-steps/s at {n // 1000}k-particle BPM Brazilian test (1 GPU)"
+    else:
+        r_max = 0.35 if wl == "uniaxial-100k" else 0.30
+        w = config.make_workload(n=n, r_min=0.20, r_max=r_max, test="uniaxial")
+        # strain rate 0.1/s: closing speed = 0.1/s * specimen height
+        v_mps = -0.1 * w["cfg"].size[2] * 1e-3
+        w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v_mps, dnsave=100)
+        desc = (f"BPM uniaxial compression, {n} spheres r~U[0.20,{r_max:.2f}] mm, "
+                f"2:1 cylinder, strain rate 0.1/s, FP64")
+        metric = f"particle-steps/s at {n // 1000}k-particle BPM uniaxial test (1 GPU)"
     log(f"sample n={w['n']} generated in {time.perf_counter() - t0:.1f}s, "
-        f"height {height_mm:.2f} mm, v_load {v_mps:.3e} m/s")
+        f"v_load {v_mps:.3e} m/s")
 
     ctx = Context(w["xyz"], w["r"], w["mat"], w["box"])
     ctx.build_bonds()
@@ -349,7 +371,7 @@ def main():
         cpu = cpu_baseline(w, ksteps, threads)
 
     out = {
-        "metric": "particle-steps/s at 100k-particle BPM uniaxial test (1 GPU)",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -361,8 +383,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded generator: jittered-FCC start, overlap relaxation)",
-        "config": {"workload": f"BPM uniaxial compression, {c.n} spheres r~U[0.20,0.35] mm, "
-                               f"2:1 cylinder, strain rate 0.1/s, FP64",
+        "config": {"workload": desc,
                    "n_particles": c.n, "partners_per_particle": c.occupied / c.n,
                    "bonds_per_particle": c.bonds / c.n,
                    "contacts_per_particle": c.contacts / c.n,
