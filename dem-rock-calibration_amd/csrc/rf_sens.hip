@@ -373,42 +373,39 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 // particle; tangents the same way per direction.
 __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t, DevState s,
                                                        Ctrl* __restrict__ ctrl) {
-    __shared__ double vb[256], vt[256];
-    __shared__ double db[kSensK][256], dt_[kSensK][256];
-    __shared__ int nb[256];
     if (ctrl->halt) return;
-    const int tid = threadIdx.x, g = tid & 31;
+    const int tid = threadIdx.x, g = tid & 31;  // one group per 32 lanes
     const int64_t grp = (int64_t)blockIdx.x * 8 + (tid >> 5);
     const int64_t i = grp * 32 + g;
     const bool in = i < p.n;
-    const int nl = t.nl;
-    vb[tid] = in ? t.pp[2 * i] : -0.0;
-    vt[tid] = in ? t.pp[2 * i + 1] : -0.0;
-    nb[tid] = in ? t.pnb[i] : 0;
-    for (int k = 0; k < nl; ++k) {
-        db[k][tid] = in ? t.tpp[(i * kSensK + k) * 2] : -0.0;
-        dt_[k][tid] = in ? t.tpp[(i * kSensK + k) * 2 + 1] : -0.0;
-    }
-    __syncthreads();
+    const bool head = g == 0 && grp < p.nblk;
+    double vb = in ? t.pp[2 * i] : -0.0, vt = in ? t.pp[2 * i + 1] : -0.0;
+    int nb = in ? t.pnb[i] : 0;
+#pragma unroll
     for (int off = 16; off >= 1; off >>= 1) {
-        if (g < off) {
-            vb[tid] = vb[tid] + vb[tid + off];
-            vt[tid] = vt[tid] + vt[tid + off];
-            nb[tid] += nb[tid + off];
-            for (int k = 0; k < nl; ++k) {
-                db[k][tid] = db[k][tid] + db[k][tid + off];
-                dt_[k][tid] = dt_[k][tid] + dt_[k][tid + off];
-            }
-        }
-        __syncthreads();
+        vb = vb + __shfl_down(vb, off, 32);
+        vt = vt + __shfl_down(vt, off, 32);
+        nb += __shfl_down(nb, off, 32);
     }
-    if (g == 0 && grp < p.nblk) {
-        s.part_b[grp] = vb[tid];
-        s.part_t[grp] = vt[tid];
-        s.part_bonds[grp] = nb[tid];
-        for (int k = 0; k < kSensK; ++k) {
-            t.tpart_b[k * p.nblk + grp] = k < nl ? db[k][tid] : 0.0;
-            t.tpart_t[k * p.nblk + grp] = k < nl ? dt_[k][tid] : 0.0;
+    if (head) {
+        s.part_b[grp] = vb;
+        s.part_t[grp] = vt;
+        s.part_bonds[grp] = nb;
+    }
+    // tangents only at the steps whose curve point the controller records
+    const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
+    if (p.kind == RF_TEST_NONE || ctrl->step % dn != 0 || ctrl->stepsave >= p.curve_cap) return;
+    for (int k = 0; k < t.nl; ++k) {
+        double db = in ? t.tpp[(i * kSensK + k) * 2] : -0.0;
+        double dt_ = in ? t.tpp[(i * kSensK + k) * 2 + 1] : -0.0;
+#pragma unroll
+        for (int off = 16; off >= 1; off >>= 1) {
+            db = db + __shfl_down(db, off, 32);
+            dt_ = dt_ + __shfl_down(dt_, off, 32);
+        }
+        if (head) {
+            t.tpart_b[k * p.nblk + grp] = db;
+            t.tpart_t[k * p.nblk + grp] = dt_;
         }
     }
 }
