@@ -216,7 +216,7 @@ def grad_bench(steps=2000, n=20000):
     grad = min(timed(True)[0] for _ in range(3))
     log(f"forward {fwd * 1e3:.1f} ms, gradient ({len(names)} directions) {grad * 1e3:.1f} ms")
     setup = dict(xyz=w["xyz"], r=w["r"], mat=mat, box=w["box"], cfg=cfg, steps=steps)
-    s, J = autograd.run_window(setup, names, theta, sens=True)
+    s0, J, e0 = autograd.run_window(setup, names, theta, sens=True, strain=True)
     # points before the first platen pause (a controller event the tangents
     # do not differentiate)
     c = Context(w["xyz"], w["r"], mat, w["box"])
@@ -229,18 +229,31 @@ def grad_bench(steps=2000, n=20000):
             npts += 1
     c.close()
     tan = 0.5 * (J[:, :, 1] - J[:, :, 0])
+    # mean axial stress and secant modulus over the curve points before the
+    # first platen pause (the window the tangents and differences share)
+    win = slice(0, npts)
+    mv, mjac = autograd.macro_from_curve(e0[win], s0[win], J[win])
     errs = {}
+    macro = {"points": npts, "mean_axial_stress_mpa": mv[0] * 1e3, "secant_modulus_gpa": mv[1],
+             "params": {}}
     for k, name in enumerate(names):
-        h = 1e-2 * abs(theta[k])
-        tp, tm = theta.copy(), theta.copy()
-        tp[k] += h
-        tm[k] -= h
-        a, _ = autograd.run_window(setup, names, tp, sens=False)
-        b, _ = autograd.run_window(setup, names, tm, sens=False)
-        fd = (0.5 * (a[:, 1] - a[:, 0]) - 0.5 * (b[:, 1] - b[:, 0])) / (2 * h)
-        sel = slice(1, npts)
-        rel = np.abs(tan[sel, k] - fd[sel]) / np.maximum(np.abs(fd[sel]), 1e-300)
-        errs[name] = {"median": float(np.median(rel)), "max": float(rel.max())}
+        entry = {"tangent": [mjac[0, k] * 1e3, mjac[1, k]]}
+        for rel in (1e-2, 1e-6):
+            h = rel * abs(theta[k])
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += h
+            tm[k] -= h
+            a, _, ea = autograd.run_window(setup, names, tp, sens=False, strain=True)
+            b, _, eb = autograd.run_window(setup, names, tm, sens=False, strain=True)
+            dm = (autograd.macro_from_curve(ea[win], a[win])[0]
+                  - autograd.macro_from_curve(eb[win], b[win])[0]) / (2 * h)
+            entry[f"central_difference_{rel:g}"] = [dm[0] * 1e3, dm[1]]
+            if rel == 1e-2:
+                fd = (0.5 * (a[:, 1] - a[:, 0]) - 0.5 * (b[:, 1] - b[:, 0])) / (2 * h)
+                sel = slice(1, npts)
+                rel_err = np.abs(tan[sel, k] - fd[sel]) / np.maximum(np.abs(fd[sel]), 1e-300)
+                errs[name] = {"median": float(np.median(rel_err)), "max": float(rel_err.max())}
+        macro["params"][name] = entry
     out = {
         "metric": "gradient cost / forward pre-peak run (20k spheres, 2000 steps, 3 directions)",
         "value": grad / fwd,
@@ -254,7 +267,8 @@ def grad_bench(steps=2000, n=20000):
                                f"{steps} steps to 0.1% strain",
                    "directions": names, "forward_ms": fwd * 1e3, "gradient_ms": grad * 1e3,
                    "fd_points": npts - 1,
-                   "tangent_vs_central_difference_1e-2": errs},
+                   "tangent_vs_central_difference_1e-2": errs,
+                   "macro_gradient": macro},
     }
     print(json.dumps(out))
 

@@ -46,9 +46,10 @@ def fixed_dt(mat, r):
     return m
 
 
-def run_window(setup, names, values, sens=True):
+def run_window(setup, names, values, sens=True, strain=False):
     """One loading window: returns (stress (points, 2) reduced units,
-    Jacobian (points, nparam, 2) or None)."""
+    Jacobian (points, nparam, 2) or None), and the curve's strain column
+    (points,) as a third value when `strain` is set."""
     mat = with_params(setup["mat"], names, values)
     c = Context(setup["xyz"], setup["r"], mat, setup["box"])
     c.build_bonds()
@@ -62,7 +63,37 @@ def run_window(setup, names, values, sens=True):
         raise RuntimeError(f"the loading window ended after {done} of {setup['steps']} steps")
     curve = c.curve()
     J = c.sens_curve() if sens else None
+    if strain:
+        return curve[:, 1:3].copy(), J, curve[:, 0].copy()
     return curve[:, 1:3].copy(), J
+
+
+def macro_from_curve(strain, stress, J=None):
+    """Mean axial stress and secant modulus of a loading window (reduced
+    units: stress and modulus in GPa), and their Jacobian when the curve's
+    is given.
+
+    strain (points,) is the curve's platen strain, (size - size0) / size0,
+    negative in compression; stress (points, 2) holds the bottom and top
+    platen stresses; J (points, nparam, 2) their derivatives.  The axial
+    stress of a point is (|s_bottom| + |s_top|) / 2 (SumForcesULoading,
+    md_results.cu:56-99); the mean runs over the window's points.  The
+    secant modulus is s_peak / e_peak (md_results.cu:149), the peak being the
+    first point of highest axial stress (the peak moves only on a strict
+    increase, md_results.cu:70-75) and e_peak = -strain there.  The peak's
+    position is an event: its derivative follows the point it picks.
+    Returns (values (2,), Jacobian (2, nparam) or None)."""
+    sa = 0.5 * (np.abs(stress[:, 0]) + np.abs(stress[:, 1]))
+    p = int(np.argmax(sa))
+    eps = -float(strain[p])
+    if not eps > 0.0:
+        raise ValueError("the window's highest stress comes before any platen travel")
+    vals = np.array([sa.mean(), sa[p] / eps])
+    if J is None:
+        return vals, None
+    dsa = 0.5 * (np.sign(stress[:, 0])[:, None] * J[:, :, 0]
+                 + np.sign(stress[:, 1])[:, None] * J[:, :, 1])
+    return vals, np.stack([dsa.mean(axis=0), dsa[p] / eps])
 
 
 class LoadingResponse(torch.autograd.Function):
@@ -83,3 +114,23 @@ class LoadingResponse(torch.autograd.Function):
         (J,) = ctx.saved_tensors
         grad = torch.einsum("pe,pke->k", g.to(J.dtype), J)
         return grad, None
+
+
+class MacroResponse(torch.autograd.Function):
+    """[mean axial stress (MPa), secant modulus (GPa)] of a loading window
+    as a function of theta (macro_from_curve over the curve of one GPU
+    sensitivity run); setup as for LoadingResponse."""
+
+    @staticmethod
+    def forward(ctx, theta, setup):
+        values = theta.detach().cpu().numpy().astype(np.float64)
+        stress, J, strain = run_window(setup, setup["names"], values, sens=True, strain=True)
+        v, jac = macro_from_curve(strain, stress, J)
+        scale = np.array([1e3, 1.0])
+        ctx.save_for_backward(torch.from_numpy(jac * scale[:, None]))
+        return torch.from_numpy(v * scale).to(theta.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        (jac,) = ctx.saved_tensors
+        return g.to(jac.dtype) @ jac, None

@@ -112,3 +112,44 @@ def test_autograd_function(base):
     np.testing.assert_allclose(th.grad.numpy(), expect, rtol=1e-13, atol=0.0)  # J^T g
     fd = (central(setup, 0, 3e-2) * 1e3 * w.numpy()).sum()
     assert abs(th.grad[0].item() - fd) <= 0.02 * abs(fd)
+
+
+def test_macro_response_against_central_differences(base):
+    """Mean axial stress and secant modulus of the 300-step window: the
+    torch Function returns macro_from_curve of its run and J^T g; the
+    derivatives of both against 3 % central differences, the step the
+    long-window test uses, within the bound that test sets on every curve
+    point (5 % of the parameter's largest pointwise difference, floored at
+    a tenth of the bond modulus'), carried through the mean and through the
+    division by the peak strain.  Friction, which acts through stick/slip
+    events, is left to that test."""
+    setup = window(base, 300)
+    th0 = theta0(setup)
+    th = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    out = autograd.MacroResponse.apply(th, setup)
+    g = torch.tensor([1.0, 0.5], dtype=torch.float64)
+    (out * g).sum().backward()
+    stress, J, strain = autograd.run_window(setup, NAMES, th0, sens=True, strain=True)
+    v, jac = autograd.macro_from_curve(strain, stress, J)
+    scale = np.array([1e3, 1.0])
+    np.testing.assert_array_equal(out.detach().numpy(), v * scale)
+    np.testing.assert_allclose(th.grad.numpy(), (g.numpy() * scale) @ jac, rtol=1e-13)
+    sa = 0.5 * (np.abs(stress[:, 0]) + np.abs(stress[:, 1]))
+    eps_peak = -strain[int(np.argmax(sa))]
+    fds, pts = [], []
+    for k in range(3):
+        h = 3e-2 * abs(th0[k])
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += h
+        tm[k] -= h
+        sp, _, ep = autograd.run_window(setup, NAMES, tp, sens=False, strain=True)
+        sm, _, em = autograd.run_window(setup, NAMES, tm, sens=False, strain=True)
+        fds.append((autograd.macro_from_curve(ep, sp)[0]
+                    - autograd.macro_from_curve(em, sm)[0]) / (2 * h))
+        pts.append(np.abs(0.5 * (np.abs(sp).sum(axis=1) - np.abs(sm).sum(axis=1)) / (2 * h)).max())
+    for k, name in enumerate(NAMES[:3]):
+        ref = max(pts[k], 0.1 * pts[0])
+        for o, r in ((0, ref), (1, ref / eps_peak)):
+            print(f"{name} output {o}: tangent {jac[o, k]:.6e} difference {fds[k][o]:.6e} "
+                  f"bound {0.05 * r:.3e}", flush=True)
+            assert abs(jac[o, k] - fds[k][o]) <= 0.05 * r, (name, o)
