@@ -860,6 +860,36 @@ rf_status rf_run(rf_ctx* c, rf_test_state* out) {
     return RF_OK;
 }
 
+static rf_status run_test(rf_ctx* c, const rf_platen_cfg* cfg, int32_t kind, const char* who,
+                          rf_result* out) {
+    if (!c || !cfg || !out) return fail(RF_ERR_ARG, "%s: null argument", who);
+    if (cfg->kind != kind) return fail(RF_ERR_ARG, "%s: cfg->kind is %d, not %d", who, cfg->kind, kind);
+    rf_status st = rf_set_test(c, cfg);
+    if (st) return st;
+    rf_test_state ts;
+    st = rf_run(c, &ts);
+    if (st) return st;
+    *out = rf_result{};
+    out->peak_stress = ts.peak_stress;
+    out->peak_strain = ts.peak_strain;
+    out->secant_modulus = ts.peak_strain != 0.0 ? ts.peak_stress / ts.peak_strain : 0.0;
+    out->linearity_deviation = ts.linearity_deviation;
+    out->destruction = ts.destruction;
+    out->steps = ts.step;
+    out->curve_points = ts.n_curve;
+    out->fracture = ts.fracture;
+    out->kind = ts.kind;
+    return RF_OK;
+}
+
+rf_status rf_run_uniaxial(rf_ctx* c, const rf_platen_cfg* cfg, rf_result* out) {
+    return run_test(c, cfg, RF_TEST_UNIAXIAL, "rf_run_uniaxial", out);
+}
+
+rf_status rf_run_brazilian(rf_ctx* c, const rf_platen_cfg* cfg, rf_result* out) {
+    return run_test(c, cfg, RF_TEST_BRAZILIAN, "rf_run_brazilian", out);
+}
+
 rf_status rf_get_test_state(rf_ctx* c, rf_test_state* out) {
     if (!c || !out) return fail(RF_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
@@ -1628,6 +1658,65 @@ rf_status rf_sens_get_curve(rf_ctx* c, double* out, int64_t cap, int64_t* n) {
             for (int e = 0; e < 2; ++e)
                 out[(q * np + k) * 2 + e] = h[((c->sens_curve0 + q) * rf::kSensK + k) * 2 + e];
     if (n) *n = cnt;
+    return RF_OK;
+}
+
+rf_status rf_adjoint_run(rf_ctx* c, const rf_adjoint_cfg* a, double* values, double* grad) {
+    if (!c || !a || !values || !grad) return fail(RF_ERR_ARG, "rf_adjoint_run: null argument");
+    if (a->steps < 1) return fail(RF_ERR_ARG, "rf_adjoint_run: the window needs at least one step");
+    if (!c->has_test || c->cfg.kind == RF_TEST_NONE)
+        return fail(RF_ERR_STATE, "rf_adjoint_run needs a platen test (rf_set_test)");
+    rf_status st = rf_sens_begin(c, a->nparam, a->params);
+    if (st) return st;
+    int64_t done = 0;
+    st = rf_sens_step(c, a->steps, &done);
+    if (st) return st;
+    if (done != a->steps)
+        return fail(RF_ERR_STATE, "rf_adjoint_run: the test ended after %lld of %lld window steps",
+                    (long long)done, (long long)a->steps);
+    const int np = a->nparam;
+    const int64_t p0 = c->sens_curve0;
+    int64_t m = 0;
+    std::vector<double> curve(3 * (size_t)std::max<int64_t>(c->p.curve_cap, 1));
+    st = rf_get_curve(c, curve.data(), c->p.curve_cap, &m);
+    if (st) return st;
+    const int64_t cnt = m - p0;
+    if (cnt < 1) return fail(RF_ERR_STATE, "rf_adjoint_run: the window recorded no curve point");
+    std::vector<double> tan((size_t)cnt * np * 2);
+    int64_t got = 0;
+    st = rf_sens_get_curve(c, tan.data(), cnt, &got);
+    if (st) return st;
+    if (got != cnt) return fail(RF_ERR_STATE, "rf_adjoint_run: curve and tangents differ in length");
+    auto sgn = [](double x) { return (double)((x > 0.0) - (x < 0.0)); };
+    double sum = 0.0, best = -1.0;
+    int64_t q = 0;
+    for (int64_t i = 0; i < cnt; ++i) {
+        const double* pt = &curve[3 * (p0 + i)];
+        const double sa = 0.5 * (std::fabs(pt[1]) + std::fabs(pt[2]));
+        sum += sa;
+        if (sa > best) {  // first point of the highest stress
+            best = sa;
+            q = i;
+        }
+    }
+    const double eps = -curve[3 * (p0 + q)];
+    if (!(eps > 0.0))
+        return fail(RF_ERR_STATE, "rf_adjoint_run: the window's highest stress comes before any "
+                                  "platen travel");
+    values[0] = sum / (double)cnt;
+    values[1] = best / eps;
+    for (int k = 0; k < np; ++k) {
+        double dsum = 0.0, dpeak = 0.0;
+        for (int64_t i = 0; i < cnt; ++i) {
+            const double* pt = &curve[3 * (p0 + i)];
+            const double* t = &tan[((size_t)i * np + k) * 2];
+            const double d = 0.5 * (sgn(pt[1]) * t[0] + sgn(pt[2]) * t[1]);
+            dsum += d;
+            if (i == q) dpeak = d;
+        }
+        grad[k] = dsum / (double)cnt;
+        grad[np + k] = dpeak / eps;
+    }
     return RF_OK;
 }
 

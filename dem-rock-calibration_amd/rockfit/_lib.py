@@ -29,7 +29,8 @@ EXPORTS = [
     "rf_fire_defaults", "rf_fire_relax", "rf_set_graph",
     "rf_sens_material_tangent", "rf_sens_begin", "rf_sens_step", "rf_sens_get_curve",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
-    "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning",
+    "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning", "rf_run_uniaxial",
+    "rf_run_brazilian", "rf_adjoint_run",
 ]
 
 
@@ -98,6 +99,12 @@ def lib():
     L.rf_sens_step.restype = st
     L.rf_sens_get_curve.argtypes = [vp, P(C.c_double), C.c_int64, P(C.c_int64)]
     L.rf_sens_get_curve.restype = st
+    L.rf_adjoint_run.argtypes = [vp, P(abi.AdjointCfg), P(C.c_double), P(C.c_double)]
+    L.rf_adjoint_run.restype = st
+    L.rf_run_uniaxial.argtypes = [vp, P(abi.PlatenCfg), P(abi.Result)]
+    L.rf_run_uniaxial.restype = st
+    L.rf_run_brazilian.argtypes = [vp, P(abi.PlatenCfg), P(abi.Result)]
+    L.rf_run_brazilian.restype = st
     L.rf_set_graph.argtypes = [vp, C.c_int32]
     L.rf_set_graph.restype = st
     L.rf_batch_create.argtypes = [P(vp), C.c_int32, P(vp)]

@@ -145,6 +145,18 @@ class Moduli(C.Structure):
                 ("r2", C.c_double), ("points", C.c_int64)]
 
 
+class Result(C.Structure):
+    _fields_ = [("peak_stress", C.c_double), ("peak_strain", C.c_double),
+                ("secant_modulus", C.c_double), ("linearity_deviation", C.c_double),
+                ("destruction", C.c_double), ("steps", C.c_int64), ("curve_points", C.c_int64),
+                ("fracture", C.c_int32), ("kind", C.c_int32)]
+
+
+class AdjointCfg(C.Structure):
+    _fields_ = [("nparam", C.c_int32), ("params", C.c_int32 * 4), ("pad_", C.c_int32),
+                ("steps", C.c_int64)]
+
+
 class FireResult(C.Structure):
     _fields_ = [
         ("steps", C.c_int64), ("np_positive", C.c_int64), ("np_negative", C.c_int64),

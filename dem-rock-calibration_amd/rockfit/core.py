@@ -126,6 +126,34 @@ class Context:
         check(lib().rf_run(self.h, C.byref(t)), "rf_run")
         return t
 
+    def run_uniaxial(self, cfg):
+        """Whole uniaxial test (rf_run_uniaxial): an abi.Result."""
+        out = abi.Result()
+        check(lib().rf_run_uniaxial(self.h, C.byref(cfg), C.byref(out)), "rf_run_uniaxial")
+        return out
+
+    def run_brazilian(self, cfg):
+        """Whole Brazilian test (rf_run_brazilian): an abi.Result."""
+        out = abi.Result()
+        check(lib().rf_run_brazilian(self.h, C.byref(cfg), C.byref(out)), "rf_run_brazilian")
+        return out
+
+    def adjoint_run(self, params, steps):
+        """Loading window of `steps` steps with tangents along the rf_param
+        directions (rf_adjoint_run): returns (values (2,), grad (2, nparam))
+        for the mean axial stress and the secant modulus, reduced units."""
+        cfg = abi.AdjointCfg()
+        cfg.nparam = len(params)
+        for k, q in enumerate(params):
+            cfg.params[k] = int(q)
+        cfg.steps = int(steps)
+        vals = np.zeros(2)
+        grad = np.zeros(2 * len(params))
+        check(lib().rf_adjoint_run(self.h, C.byref(cfg), vals.ctypes.data_as(C.POINTER(C.c_double)),
+                                   grad.ctypes.data_as(C.POINTER(C.c_double))), "rf_adjoint_run")
+        self._sens_np = len(params)
+        return vals, grad.reshape(2, len(params))
+
     def test_state(self):
         t = abi.TestState()
         check(lib().rf_get_test_state(self.h, C.byref(t)), "rf_get_test_state")

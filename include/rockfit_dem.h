@@ -213,6 +213,33 @@ rf_status rf_step(rf_ctx* ctx, int64_t nsteps, int64_t* done_steps);
  * post-process (CalculateLinearityDeviation, CalculateDestruction). */
 rf_status rf_run(rf_ctx* ctx, rf_test_state* out);
 
+/* Results of one platen test in the form the reference's drivers leave them
+ * (mdTD.Compress.FractureStress, FractureStrain, LinearityDeviation and
+ * mdTD.R.Destruction, read back at mp_calculateonesample.cu:164-169), in
+ * reduced units. */
+typedef struct rf_result {
+    double peak_stress;          /* UCS (uniaxial) or tensile strength (Brazilian) */
+    double peak_strain;
+    double secant_modulus;       /* peak_stress / peak_strain (md_results.cu:149)  */
+    double linearity_deviation;  /* CalculateLinearityDeviation                    */
+    double destruction;          /* CalculateDestruction                           */
+    int64_t steps;               /* loading steps executed                         */
+    int64_t curve_points;
+    int32_t fracture;            /* the test ended on a fracture criterion         */
+    int32_t kind;                /* rf_test_kind                                   */
+} rf_result;
+
+/* One whole uniaxial compression test on a built context: rf_set_test with
+ * cfg (whose kind must be RF_TEST_UNIAXIAL), rf_run, results.  Replaces
+ * h_CalculationUniaxialCompression(md_task_data&, float* h_R) (mp.h:16-22,
+ * md_calculationuniaxialcompression.cu:126-339), which allocates, builds and
+ * runs in one call; here rf_create + rf_build_bonds come first, so one
+ * built sample can be tested more than once. */
+rf_status rf_run_uniaxial(rf_ctx* ctx, const rf_platen_cfg* cfg, rf_result* out);
+/* The same for the Brazilian test (cfg->kind RF_TEST_BRAZILIAN); replaces
+ * h_CalculationBrazilTest (mp.h:16-22, md_calculationbraziltest.cu:116-264). */
+rf_status rf_run_brazilian(rf_ctx* ctx, const rf_platen_cfg* cfg, rf_result* out);
+
 rf_status rf_get_test_state(rf_ctx* ctx, rf_test_state* out);
 /* Curve samples: 3 doubles each (strain, bottom stress, top stress), as
  * SaveSumForcesLoading (md_results.cu:168-180) in reduced units. */
@@ -308,6 +335,28 @@ rf_status rf_sens_step(rf_ctx* ctx, int64_t nsteps, int64_t* done_steps);
 /* Tangents of the curve points recorded since rf_sens_begin:
  * out[(point*nparam + k)*2 + {0,1}] = d(bottom, top platen stress)/d(param k). */
 rf_status rf_sens_get_curve(rf_ctx* ctx, double* out, int64_t cap, int64_t* n);
+
+/* Gradient of a loading window's macroscopic response in one call:
+ * rf_sens_begin(params), rf_sens_step(steps), then over the curve points the
+ * window recorded, with s_p = (|bottom| + |top|) / 2 the axial stress of
+ * point p (SumForcesULoading, md_results.cu:56-99):
+ *   values[0] = mean of s_p (mean axial stress),
+ *   values[1] = s_q / e_q, the secant modulus at the first point q of highest
+ *               s_p, e_q = -strain there (md_results.cu:149),
+ *   grad[o * nparam + k] = d values[o] / d params[k].
+ * The derivatives come from forward-mode tangents of the same run (one run
+ * for all parameters), not from a reverse sweep; they replace the
+ * forward-difference Jacobian of CalculateStepCG
+ * (mp_determineconjgradient.cu:84-142).  The context needs a platen test
+ * (rf_set_test); the window starts at its current step.  RF_ERR_STATE if
+ * the test ends inside the window or the window records no loaded point. */
+typedef struct rf_adjoint_cfg {
+    int32_t nparam;                 /* 1..RF_SENS_MAX                          */
+    int32_t params[RF_SENS_MAX];    /* rf_param values                         */
+    int32_t pad_;
+    int64_t steps;                  /* loading steps of the window             */
+} rf_adjoint_cfg;
+rf_status rf_adjoint_run(rf_ctx* ctx, const rf_adjoint_cfg* cfg, double* values, double* grad);
 
 /* ------------------------------------------------------------------------
  * Batches: several samples advanced together, one step launch and one

@@ -97,6 +97,8 @@ int64_t ora_sizeof(int32_t which) {
         case 8: return (int64_t)sizeof(rf_state_view);
         case 9: return (int64_t)sizeof(rf_fire_cfg);
         case 10: return (int64_t)sizeof(rf_fire_result);
+        case 11: return (int64_t)sizeof(rf_result);
+        case 12: return (int64_t)sizeof(rf_adjoint_cfg);
         default: return -1;
     }
 }

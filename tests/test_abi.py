@@ -35,7 +35,8 @@ def test_library_exports_every_declared_symbol():
                                        (3, abi.Options), (4, abi.Derived), (5, abi.PlatenCfg),
                                        (6, abi.TestState), (7, abi.Counters),
                                        (8, abi.StateView), (9, abi.FireCfg),
-                                       (10, abi.FireResult)])
+                                       (10, abi.FireResult), (11, abi.Result),
+                                       (12, abi.AdjointCfg)])
 def test_struct_layout(which, cls):
     assert pyoracle.lib().ora_sizeof(which) == C.sizeof(cls)
 

@@ -19,6 +19,7 @@ import torch
 
 from conftest import gpu_available
 from rockfit import autograd, config
+from rockfit.core import Context
 
 pytestmark = pytest.mark.gpu
 
@@ -153,3 +154,20 @@ def test_macro_response_against_central_differences(base):
             print(f"{name} output {o}: tangent {jac[o, k]:.6e} difference {fds[k][o]:.6e} "
                   f"bound {0.05 * r:.3e}", flush=True)
             assert abs(jac[o, k] - fds[k][o]) <= 0.05 * r, (name, o)
+
+
+def test_adjoint_run_equals_macro_from_curve(base):
+    """rf_adjoint_run (one C-ABI call) gives the values and Jacobian that
+    macro_from_curve computes from the same window's curve and tangents."""
+    setup = window(base, 300)
+    th0 = theta0(setup)
+    stress, J, strain = autograd.run_window(setup, NAMES, th0, sens=True, strain=True)
+    v, jac = autograd.macro_from_curve(strain, stress, J)
+    mat = autograd.with_params(setup["mat"], NAMES, th0)
+    c = Context(setup["xyz"], setup["r"], mat, setup["box"])
+    c.build_bonds()
+    c.set_test(setup["cfg"])
+    vals, grad = c.adjoint_run([autograd.PARAMS[n][0] for n in NAMES], setup["steps"])
+    np.testing.assert_allclose(vals, v, rtol=1e-12)
+    np.testing.assert_allclose(grad, jac, rtol=1e-12, atol=1e-12 * np.abs(jac).max())
+    c.close()
