@@ -221,3 +221,37 @@ def test_ejected_particles_parity():
     assert g.step(500) == o.step(500) == 500
     compare_states(g.get_state(), o.get_state(), n, g.slots, what="ejected, parked")
     assert g.counters().rebuilds - before < 50
+
+
+@pytest.mark.parametrize("kind", ["uniaxial", "brazilian"])
+def test_full_run_macro_parity(kind):
+    """A whole test, loading at 0.5 m/s until the fracture criterion ends it,
+    through the drop-in entry point (rf_run_uniaxial / rf_run_brazilian)
+    against the oracle run to the same end: the macroscopic results (peak
+    stress, i.e. UCS or tensile strength, peak strain, secant modulus,
+    linearity deviation, destruction), the step count, the whole curve and
+    the bond-count trace are identical, so their error is 0."""
+    print(f"case full {kind} run n=1500 v=-0.5 m/s", flush=True)
+    shape, aspect = ("cylinder", 2.0) if kind == "uniaxial" else ("disc", 0.5)
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test=kind, shape=shape, aspect=aspect,
+                             seed=12, test_kwargs=dict(v_load_mps=-0.5, dnsave=20, max_steps=30000))
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    res = g.run_uniaxial(w["cfg"]) if kind == "uniaxial" else g.run_brazilian(w["cfg"])
+    o.set_test(w["cfg"])
+    o.step(30000)
+    o.post()
+    t = o.test_state()
+    assert t.done == 1 and t.fracture == 1 and res.fracture == 1
+    print(f"{kind}: peak {res.peak_stress * 1e3:.4f} MPa at strain {res.peak_strain:.6f}, "
+          f"secant {res.secant_modulus:.4f} GPa, LD {res.linearity_deviation:.6f}, "
+          f"destruction {res.destruction:.6f}, {res.steps} steps", flush=True)
+    assert res.peak_stress == t.peak_stress and res.peak_strain == t.peak_strain
+    assert res.secant_modulus == t.peak_stress / t.peak_strain
+    assert res.linearity_deviation == t.linearity_deviation
+    assert res.destruction == t.destruction
+    assert res.steps == t.step and res.curve_points == t.n_curve
+    np.testing.assert_array_equal(g.curve(), o.curve())
+    np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
+    compare_test_states(g.test_state(), t, what=f"end of the {kind} run")
