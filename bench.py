@@ -156,6 +156,13 @@ def batch_bench(count, steps, warmup):
     pst_seq = total_n * (seq_steps / count) / t_seq
     log(f"batched {t_bat:.3f}s ({bat_steps} sample-steps), sequential {t_seq:.3f}s "
         f"({seq_steps} sample-steps)")
+    # loading steps of one whole test of each sample: the platens close at
+    # |V| to the strain limit (the short approach at 10 V left out)
+    from rockfit.core import derive
+    test_steps = [w["cfg"].max_strain * w["cfg"].size[2]
+                  / (abs(w["cfg"].v_load) * derive(w["mat"], float(w["r"].min()),
+                                                   float(w["r"].max())).dt) for w in ws]
+    mean_test_steps = float(np.mean(test_steps))
     print(json.dumps({
         "metric": f"particle-steps/s, {count}-sample batch (N log-U[2k,40k]) on 1 GPU",
         "value": pst_bat, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
@@ -167,6 +174,12 @@ def batch_bench(count, steps, warmup):
                        "kind": "same samples one after another on the GPU"},
         "speedup": pst_bat / pst_seq,
         "sample_steps_per_s": {"batched": bat_steps / t_bat, "sequential": seq_steps / t_seq},
+        "samples_per_hour": {
+            "batched": 3600.0 * bat_steps / t_bat / mean_test_steps,
+            "sequential": 3600.0 * seq_steps / t_seq / mean_test_steps,
+            "steps_per_test": mean_test_steps,
+            "basis": "3600 x sample-steps/s / mean loading steps of one test "
+                     "(max strain 1 % at strain rate 0.1/s: max_strain H / (|V| dt))"},
         "block_order": {"chosen": ["by sample", "interleaved", "largest first",
                                    "one after another"][order]
                         if order >= 0 else None, "ms_per_step": tune_ms},
