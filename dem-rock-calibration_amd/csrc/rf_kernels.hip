@@ -720,7 +720,10 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     __shared__ long long sn[kCtrlThreads];
     const int t = threadIdx.x;
     // the partials are read before the halt flag is known (a halted step
-    // leaves them as they were), so both loads are in flight together
+    // leaves them as they were), so these loads and the control block's
+    // (uniform: scalar loads) are in flight together; nothing in this
+    // kernel writes the control block before its last store
+    const Ctrl c0 = *ctrl;
     double ab = 0.0, at = 0.0;
     long long nb = 0;
 #pragma unroll 4
@@ -729,7 +732,7 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
         at = at + s.part_t[b];
         nb += s.part_bonds[b];
     }
-    if (ctrl->halt) return;
+    if (c0.halt) return;
     // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
     // one shuffle per level
     sb[t] = ab;
@@ -757,7 +760,7 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     sb[0] = vb;
     stt[0] = vt;
     sn[0] = vn;
-    Ctrl c = *ctrl;
+    Ctrl c = c0;
     if (p.kind != RF_TEST_NONE) {
         c.sfl0 = sb[0];
         c.sfl1 = stt[0];
