@@ -86,6 +86,7 @@ struct rf_ctx {
     rf_derived der{};
     rf_box box{};
     rf_options opt{};
+    bool own_stream = true;  // created by rf_create (else the caller's, rf_options.stream)
     rf::DevState d{};
     rf::DevParams p{};
     rf::Ctrl* ctrl = nullptr;
@@ -210,6 +211,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
         if (opt->cand_cap > 0) o.cand_cap = opt->cand_cap;
         if (opt->skin >= 0.0) o.skin = opt->skin;
         o.device = opt->device;
+        o.stream = opt->stream;
     }
     if (o.slots > rf::kMaxSlots)
         return fail(RF_ERR_ARG, "slots=%d exceeds the %d lanes of a particle group", o.slots,
@@ -253,10 +255,18 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     const int64_t n = c->n;
     auto bail = [&](rf_status st) {
         for (void* ptr : c->allocs) hipFree(ptr);
+        if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
         delete c;
         return st;
     };
-    if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(RF_ERR_HIP, "hipStreamCreate"));
+    if (o.stream) {
+        c->stream = (hipStream_t)o.stream;
+        c->own_stream = false;
+    } else {
+        if (hipStreamCreate(&c->stream) != hipSuccess)
+            return bail(fail(RF_ERR_HIP, "hipStreamCreate"));
+        c->own_stream = true;
+    }
     rf::DevState& d = c->d;
     const int64_t nS = n * c->S, nC = n * c->C, nB = n * c->B;
     rf_status st = RF_OK;
@@ -380,7 +390,7 @@ rf_status rf_destroy(rf_ctx* c) {
     for (auto& g : c->graphs)
         if (g.exec) hipGraphExecDestroy(g.exec);
     for (void* ptr : c->allocs) hipFree(ptr);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
     delete c;
     return RF_OK;
 }

@@ -60,7 +60,7 @@ class Sample(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("slots", C.c_int32), ("cand_cap", C.c_int32), ("skin", C.c_double),
-                ("device", C.c_int32), ("block", C.c_int32)]
+                ("device", C.c_int32), ("block", C.c_int32), ("stream", C.c_void_p)]
 
 
 class Derived(C.Structure):

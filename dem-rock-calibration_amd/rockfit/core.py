@@ -80,7 +80,10 @@ def curve_moduli(curve, peak_strain, peak_stress, lo=0.1, hi=0.5):
 class Context:
     """One sample on the GPU."""
 
-    def __init__(self, xyz, r, mat, box, slots=32, cand_cap=32, skin=-1.0, device=0):
+    def __init__(self, xyz, r, mat, box, slots=32, cand_cap=32, skin=-1.0, device=0,
+                 stream=None):
+        """stream: a hipStream_t (as an int handle, e.g. torch.cuda.Stream's
+        cuda_stream) the context runs on instead of its own."""
         self.n = int(len(r))
         self.slots = int(slots)
         self._x = np.ascontiguousarray(xyz[:, 0], dtype=np.float64)
@@ -90,7 +93,7 @@ class Context:
         dp = C.POINTER(C.c_double)
         s = abi.Sample(self.n, self._x.ctypes.data_as(dp), self._y.ctypes.data_as(dp),
                        self._z.ctypes.data_as(dp), self._r.ctypes.data_as(dp))
-        opt = abi.Options(slots, cand_cap, skin, device, 256)
+        opt = abi.Options(slots, cand_cap, skin, device, 256, stream)
         h = C.c_void_p()
         self.h = None
         check(lib().rf_create(C.byref(s), C.byref(mat), C.byref(box), C.byref(opt), C.byref(h)),

@@ -104,6 +104,10 @@ typedef struct rf_options {
                              when a list overflows); never changes results */
     int32_t device;       /* HIP device ordinal                                */
     int32_t block;        /* threads per block of the per-particle kernels     */
+    void* stream;         /* hipStream_t the context runs on: one the caller
+                             created on `device` (not the null stream), kept
+                             by the caller and not destroyed by rf_destroy;
+                             NULL: the context creates and owns its stream */
 } rf_options;
 
 /* Derived parameters (potential_data::CalculateParameters, md_data_types.h:155-187). */

@@ -255,3 +255,29 @@ def test_full_run_macro_parity(kind):
     np.testing.assert_array_equal(g.curve(), o.curve())
     np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
     compare_test_states(g.test_state(), t, what=f"end of the {kind} run")
+
+
+def test_caller_stream():
+    """A context on a stream the caller made (rf_options.stream) runs the same
+    steps as one on its own stream, bit for bit, and rf_destroy leaves that
+    stream to the caller."""
+    import torch
+
+    print("case caller stream n=1500", flush=True)
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="uniaxial", seed=9,
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=5, max_steps=600))
+    st = torch.cuda.Stream()
+    a = Context(w["xyz"], w["r"], w["mat"], w["box"], stream=st.cuda_stream)
+    b = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    for c in (a, b):
+        c.build_bonds()
+        c.set_test(w["cfg"])
+        assert c.step(600) == 600
+    compare_states(a.get_state(), b.get_state(), a.n, a.slots, what="caller stream")
+    np.testing.assert_array_equal(a.curve(), b.curve())
+    a.close()
+    b.close()
+    with torch.cuda.stream(st):  # still usable after rf_destroy
+        x = torch.ones(1000, device="cuda", dtype=torch.float64).sum()
+    st.synchronize()
+    assert x.item() == 1000.0
