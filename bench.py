@@ -359,9 +359,13 @@ def main():
     log(f"sample n={w['n']} generated in {time.perf_counter() - t0:.1f}s, "
         f"v_load {v_mps:.3e} m/s")
 
-    ctx = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    device_sync()
+    t_setup = time.perf_counter()
+    ctx = Context(w["xyz"], w["r"], w["mat"], w["box"])  # host arrays copied in
     ctx.build_bonds()
     ctx.set_test(w["cfg"])
+    device_sync()
+    t_setup = time.perf_counter() - t_setup
     c0 = ctx.counters()
     log(f"built: partners/particle {c0.occupied / c0.n:.2f}, bonds/particle "
         f"{c0.bonds / c0.n:.2f}, candidates/particle {c0.candidates / c0.n:.2f}")
@@ -376,6 +380,10 @@ def main():
     done = ctx.step(args.steps)
     device_sync()
     wall = time.perf_counter() - t1
+    t_read = time.perf_counter()
+    ts_end = ctx.test_state()  # results back to the host
+    curve_end = ctx.curve()
+    t_read = time.perf_counter() - t_read
     kernel_ms, timed, total_ms, steps = ctx.last_timing()
     c = ctx.counters()
     if done != args.steps:
@@ -418,6 +426,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(c.n)},
         "cpu_baseline": cpu,
+        "host_inclusive": {
+            "value": c.n * done / (t_setup + wall + t_read), "unit": "particle-steps/s",
+            "setup_s": t_setup, "readback_s": t_read, "curve_points": int(len(curve_end)),
+            "step": int(ts_end.step),
+            "what": "rf_create from host arrays, rf_build_bonds and rf_set_test, then the timed "
+                    "steps, then the test state and curve read back, over the timed steps"},
     }
     print(json.dumps(out))
 
