@@ -181,7 +181,7 @@ def batch_bench(count, steps, warmup):
             "basis": "3600 x sample-steps/s / mean loading steps of one test "
                      "(max strain 1 % at strain rate 0.1/s: max_strain H / (|V| dt))"},
         "block_order": {"chosen": ["by sample", "interleaved", "largest first",
-                                   "one after another"][order]
+                                   "one after another", "concurrent streams"][order]
                         if order >= 0 else None, "ms_per_step": tune_ms},
     }))
 

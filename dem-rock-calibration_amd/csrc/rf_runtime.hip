@@ -423,7 +423,7 @@ rf_status device_error(rf_ctx* c) {
 // read the step's buffers and halt flag from the device control block, so the
 // same graph serves every chunk of its length; it is captured again only when
 // the constants passed by value (DevParams) change.
-rf_status graph_chunk(rf_ctx* c, int64_t k) {
+rf_status ensure_graph(rf_ctx* c, int64_t k, rf_ctx::Graph** out) {
     rf_ctx::Graph* g = nullptr;
     for (auto& q : c->graphs)
         if (q.steps == k) g = &q;
@@ -452,7 +452,28 @@ rf_status graph_chunk(rf_ctx* c, int64_t k) {
         }
         g->p = c->p;
     }
+    *out = g;
+    return RF_OK;
+}
+
+// A chunk of k steps captured once as a graph (per chunk length, re-captured
+// when the constants it baked in change) and replayed.
+rf_status graph_chunk(rf_ctx* c, int64_t k) {
+    rf_ctx::Graph* g = nullptr;
+    rf_status st = ensure_graph(c, k, &g);
+    if (st) return st;
     HIP_TRY(hipGraphLaunch(g->exec, c->stream));
+    return RF_OK;
+}
+
+// k loading steps of one sample on its stream, without waiting: a captured
+// graph (one per chunk length) or, for a single step, the two launches.
+rf_status launch_chunk(rf_ctx* c, int64_t k) {
+    if (c->use_graph && k > 1) return graph_chunk(c, k);
+    for (int64_t s = 0; s < k; ++s) {
+        HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
+        HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
+    }
     return RF_OK;
 }
 
@@ -1162,13 +1183,16 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
 // first, or no shared launch at all -- each sample stepped on its own
 // stream, one after another, as rf_step does (large samples fill the GPU
 // alone, and the shared launch's per-block table lookups then cost more
-// than they save).  Full chunks replay as captured graphs (their kernels
+// than they save), or each sample on its own stream with every sample's
+// chunk launched before any is waited for (the streams overlap on the
+// device's queues).  Full chunks replay as captured graphs (their kernels
 // read the tables and entries from device memory, so one graph per chunk
 // length serves every order), and chunks shorten while list rebuilds come
 // often.
 
-constexpr int kBatchOrders = 4;       // three block orders and one after another
+constexpr int kBatchOrders = 5;       // three block orders, one after another, concurrent
 constexpr int kBatchSequential = 3;
+constexpr int kBatchConcurrent = 4;
 constexpr int kTuneSteps = 32;
 
 struct rf_batch {
@@ -1184,7 +1208,7 @@ struct rf_batch {
     int32_t order = -1;      // table order in use
     int32_t chosen = -1;     // order picked by the tuner (-1: not yet)
     int32_t tuning = 0;      // next order to time
-    double tune_ms[kBatchOrders] = {0.0, 0.0, 0.0, 0.0};
+    double tune_ms[kBatchOrders] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double last_ms = 0.0;
     struct Graph {
         int64_t steps = 0;
@@ -1381,7 +1405,8 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
         const bool tune = b->chosen < 0;
         const int mode = tune ? b->tuning : b->chosen;
         const bool seq = mode == kBatchSequential;
-        if (tune && !seq) {
+        const bool conc = mode == kBatchConcurrent;
+        if (tune && !seq && !conc) {
             st = batch_order(b, b->tuning);
             if (st) { result = st; break; }
         }
@@ -1392,6 +1417,17 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
             before[q] = b->c[q]->hctrl.step;
             reb[q] = b->c[q]->rebuilds;
             idle[q] = b->c[q]->hctrl.halt != 0;
+        }
+        if (conc && k > 1) {  // graphs captured before the chunk (and its timing) starts
+            for (size_t q = 0; q < count && !result; ++q) {
+                rf_ctx* c = b->c[q];
+                const int64_t left = stop[q] - c->hctrl.step;
+                if (idle[q] || c->hctrl.done || left <= 0 || !c->use_graph) continue;
+                rf_ctx::Graph* g = nullptr;
+                st = ensure_graph(c, std::min<int64_t>(k, left), &g);
+                if (st) result = st;
+            }
+            if (result) break;
         }
         const auto ta = clock::now();
         bool progress = false, rebuilt = false, full = true;
@@ -1413,6 +1449,39 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
                 full = full && (idle[q] || c->hctrl.step - before[q] == k);
                 rebuilt = rebuilt || c->rebuilds != reb[q];
             }
+        } else if (conc) {
+            // every active sample's chunk launched on its own stream before
+            // any is waited for, so the samples' launches overlap on the device
+            std::vector<char> launched(count, 0);
+            for (size_t q = 0; q < count; ++q) {
+                rf_ctx* c = b->c[q];
+                if (idle[q] || c->hctrl.done) continue;
+                const int64_t left = stop[q] - c->hctrl.step;
+                if (left <= 0) continue;
+                st = launch_chunk(c, std::min<int64_t>(k, left));
+                if (st) { result = st; break; }
+                launched[q] = 1;
+            }
+            for (size_t q = 0; q < count; ++q) {  // after an error too: drain what was launched
+                if (!launched[q]) continue;
+                st = sync_ctrl(b->c[q]);
+                if (st && !result) result = st;
+            }
+            if (result) break;
+            for (size_t q = 0; q < count && !result; ++q) {
+                rf_ctx* c = b->c[q];
+                progress = progress || c->hctrl.step != before[q];
+                full = full && (idle[q] || c->hctrl.step - before[q] == k);
+                if (!launched[q]) continue;
+                if (c->hctrl.err) { result = device_error(c); break; }
+                if (c->hctrl.rebuild) {
+                    st = rebuild(c);
+                    if (st) { result = st; break; }
+                    if (c->hctrl.err) { result = device_error(c); break; }
+                    rebuilt = true;
+                }
+            }
+            if (result) break;
         } else {
             st = batch_launch(b, k, !tune && k > 1);
             if (st) { result = st; break; }
@@ -1440,7 +1509,7 @@ rf_status rf_batch_step(rf_batch* b, int64_t nsteps, int64_t* done_steps) {
                 for (int o = 1; o < kBatchOrders; ++o)
                     if (b->tune_ms[o] < b->tune_ms[best]) best = o;
                 b->chosen = best;
-                if (best != kBatchSequential) {
+                if (best != kBatchSequential && best != kBatchConcurrent) {
                     st = batch_order(b, best);
                     if (st) { result = st; break; }
                 }

@@ -279,8 +279,9 @@ class Batch:
     def tuning(self):
         """(choice or -1, [ms per step of each way]): 0-2 one shared launch
         with the blocks by sample, interleaved, largest sample first; 3 each
-        sample alone, one after another."""
+        sample alone, one after another; 4 each sample on its own stream,
+        all launched before any is waited for."""
         order = C.c_int32()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * 5)()
         check(lib().rf_batch_tuning(self.h, C.byref(order), ms), "rf_batch_tuning")
         return order.value, list(ms)

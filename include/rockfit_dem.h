@@ -383,12 +383,14 @@ rf_status rf_batch_run(rf_batch* batch, rf_test_state* out);
 /* Wall time (ms) of the last rf_batch_step / rf_batch_run. */
 rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
 /* The batch's launch autotuner (no reference counterpart): the first steps of
- * a batch run in short chunks in each of four ways -- one shared launch per
+ * a batch run in short chunks in each of five ways -- one shared launch per
  * step with the blocks ordered by sample (0), interleaved (1) or largest
- * sample first (2), or each sample stepped alone, one after another (3) --
- * timed on the host, and the fastest is kept.  The choice never changes a
- * result.  *order receives the choice (-1 before tuning), ms_per_step
- * (4 entries, may be null) the measured step times. */
+ * sample first (2), each sample stepped alone, one after another (3), or
+ * each sample on its own stream with all chunks launched before any is
+ * waited for (4) -- timed on the host, and the fastest is kept.  The choice
+ * never changes a result.  *order receives the choice (-1 before tuning),
+ * ms_per_step (RF_BATCH_WAYS entries, may be null) the measured step times. */
+#define RF_BATCH_WAYS 5
 rf_status rf_batch_tuning(rf_batch* batch, int32_t* order, double* ms_per_step);
 
 /* ------------------------------------------------------------------------
