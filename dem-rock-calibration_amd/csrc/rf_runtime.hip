@@ -1177,7 +1177,7 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
 //
 // Autotuning: the order of the block table is free (a block's work depends
 // only on its sample and its local index).  The first steps of a batch run
-// short chunks in each of four ways, timed on the host clock around the
+// short chunks in each of five ways, timed on the host clock around the
 // synchronised chunk, and the fastest is kept: one launch per step with the
 // samples' blocks one sample after another, interleaved, or largest sample
 // first, or no shared launch at all -- each sample stepped on its own
