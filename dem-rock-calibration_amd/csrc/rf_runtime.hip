@@ -1567,6 +1567,26 @@ rf_status rf_batch_tuning(rf_batch* b, int32_t* order, double* ms_per_step) {
     return RF_OK;
 }
 
+rf_status rf_batch_set_form(rf_batch* b, int32_t form) {
+    if (!b || form < -1 || form >= kBatchOrders)
+        return fail(RF_ERR_ARG, "rf_batch_set_form: form %d not in -1..%d", form, kBatchOrders - 1);
+    HIP_TRY(hipSetDevice(b->device));
+    if (form < 0) {  // back to the autotuner
+        b->chosen = -1;
+        b->tuning = 0;
+        for (int o = 0; o < kBatchOrders; ++o) b->tune_ms[o] = 0.0;
+        return RF_OK;
+    }
+    b->chosen = form;
+    if (form != kBatchSequential && form != kBatchConcurrent) {
+        rf_status st = batch_order(b, form);
+        if (st) return st;
+        st = batch_sync(b);
+        if (st) return st;
+    }
+    return RF_OK;
+}
+
 rf_status rf_batch_last_ms(rf_batch* b, double* total_ms) {
     if (!b || !total_ms) return fail(RF_ERR_ARG, "null argument");
     *total_ms = b->last_ms;

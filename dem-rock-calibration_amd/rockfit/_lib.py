@@ -30,7 +30,7 @@ EXPORTS = [
     "rf_sens_material_tangent", "rf_sens_begin", "rf_sens_step", "rf_sens_get_curve",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
     "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning", "rf_run_uniaxial",
-    "rf_run_brazilian", "rf_adjoint_run",
+    "rf_run_brazilian", "rf_adjoint_run", "rf_batch_set_form",
 ]
 
 
@@ -119,6 +119,8 @@ def lib():
     L.rf_batch_last_ms.restype = st
     L.rf_batch_tuning.argtypes = [vp, P(C.c_int32), P(C.c_double)]
     L.rf_batch_tuning.restype = st
+    L.rf_batch_set_form.argtypes = [vp, C.c_int32]
+    L.rf_batch_set_form.restype = st
     L.rf_fire_defaults.argtypes = [P(abi.FireCfg)]
     L.rf_fire_defaults.restype = st
     L.rf_fire_relax.argtypes = [C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),

@@ -276,6 +276,11 @@ class Batch:
         check(lib().rf_batch_last_ms(self.h, C.byref(ms)), "rf_batch_last_ms")
         return ms.value
 
+    def set_form(self, form):
+        """Fix the launch form (0-4, as in tuning()) or return to the
+        autotuner (-1)."""
+        check(lib().rf_batch_set_form(self.h, int(form)), "rf_batch_set_form")
+
     def tuning(self):
         """(choice or -1, [ms per step of each way]): 0-2 one shared launch
         with the blocks by sample, interleaved, largest sample first; 3 each

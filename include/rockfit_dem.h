@@ -391,6 +391,9 @@ rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
  * never changes a result.  *order receives the choice (-1 before tuning),
  * ms_per_step (RF_BATCH_WAYS entries, may be null) the measured step times. */
 #define RF_BATCH_WAYS 5
+/* Fix the launch form (0..RF_BATCH_WAYS-1, as listed above) instead of
+ * tuning it, or return to the autotuner (-1).  Results do not depend on it. */
+rf_status rf_batch_set_form(rf_batch* batch, int32_t form);
 rf_status rf_batch_tuning(rf_batch* batch, int32_t* order, double* ms_per_step);
 
 /* ------------------------------------------------------------------------

@@ -52,15 +52,22 @@ def contexts(ws, skins):
     return out
 
 
-def test_batch_steps_equal_single_runs():
+@pytest.mark.parametrize("form", [-1, 1, 3, 4])
+def test_batch_steps_equal_single_runs(form):
+    """form -1: the autotuner (every form runs during its first chunks);
+    1: one shared launch, blocks interleaved; 3: samples one after another;
+    4: samples on concurrent streams."""
     ws = workloads()
     skins = [-1.0, 0.02, -1.0, 0.01]
     batch_ctx = contexts(ws, skins)
     single_ctx = contexts(ws, skins)
     b = Batch(batch_ctx)
+    if form >= 0:
+        b.set_form(form)
+        assert b.tuning()[0] == form
     total = [0] * len(ws)
     for chunk in (1, 37, 300):
-        print(f"batch chunk {chunk}", flush=True)
+        print(f"batch form {form} chunk {chunk}", flush=True)
         done = b.step(chunk)
         for k, c in enumerate(single_ctx):
             assert c.step(chunk) == done[k]
