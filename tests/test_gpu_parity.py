@@ -281,3 +281,31 @@ def test_caller_stream():
         x = torch.ones(1000, device="cuda", dtype=torch.float64).sum()
     st.synchronize()
     assert x.item() == 1000.0
+
+
+def test_full_size_parity():
+    """The benchmark's workload at its full size: 100,000 spheres,
+    r ~ U[0.20, 0.35] mm, 2:1 cylinder, uniaxial loading at strain rate
+    0.1/s.  Build and 300 loading steps (the platens close in and start to
+    load the specimen) equal the oracle's bit for bit: particle state,
+    slot table, bond numbering, histories, controller and curve."""
+    import os
+
+    print("case full size n=100000 300 steps", flush=True)
+    w = config.make_workload(n=100000, r_min=0.20, r_max=0.35, test="uniaxial")
+    w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=-0.1 * w["cfg"].size[2] * 1e-3,
+                                   dnsave=10)
+    g = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"],
+                        threads=min(16, os.cpu_count() or 1))
+    g.build_bonds()
+    o.build_bonds()
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="full size after build")
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    assert g.step(300) == o.step(300) == 300
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="full size, 300 steps")
+    compare_test_states(g.test_state(), o.test_state(), what="full size, 300 steps")
+    np.testing.assert_array_equal(g.curve(), o.curve())
+    np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
+    print(f"stress after 300 steps {g.test_state().stress * 1e3:.6f} MPa", flush=True)
