@@ -115,6 +115,9 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 #ifndef RF_BOND_SPREAD
 #define RF_BOND_SPREAD 1
 #endif
+#ifndef RF_CAND_TAIL_LATE
+#define RF_CAND_TAIL_LATE 1
+#endif
 #ifndef RF_STEP_WAVES_PER_EU
 #define RF_STEP_WAVES_PER_EU 0
 #endif
@@ -210,13 +213,22 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const int nbd = valid ? nbd_raw : 0;
     int32_t jk[kPerLane], cj[kPerLane];
     int cs[kPerLane];
+    // Candidate entries 8.. exist only in particles with more than 8
+    // candidates (a few per cent): those are loaded below, once the counts
+    // are in, and only in the waves that have such a particle; until then
+    // they read as absent (entries past the count are never used).
 #pragma unroll
     for (int t = 0; t < kPerLane; ++t) {
         const int k = l + kLanes * t;
         jk[t] = s.nbr[ic * S + (k < S ? k : S - 1)];
-        const int64_t qc = ic * C + (k < C ? k : C - 1);
-        cj[t] = s.cand[qc];
-        cs[t] = s.cslot[qc];
+        if (t == 0 || !RF_CAND_TAIL_LATE) {
+            const int64_t qc = ic * C + (k < C ? k : C - 1);
+            cj[t] = s.cand[qc];
+            cs[t] = s.cslot[qc];
+        } else {
+            cj[t] = (int32_t)ic;
+            cs[t] = -1;
+        }
     }
     const int64_t qb0 = ic * B + l;
     const int bs0 = s.bslot[qb0];
@@ -225,6 +237,18 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const int64_t qb1 = ic * B + (l + kLanes < B ? l + kLanes : B - 1);
     const int bs1 = s.bslot[qb1];
     const int32_t bp1 = s.bpart[qb1];
+
+#if RF_CAND_TAIL_LATE
+    if (__any(nc > kLanes)) {  // wave-uniform; rare
+#pragma unroll
+        for (int t = 1; t < kPerLane; ++t) {
+            const int k = l + kLanes * t;
+            const int64_t qc = ic * C + (k < C ? k : C - 1);
+            cj[t] = s.cand[qc];
+            cs[t] = s.cslot[qc];
+        }
+    }
+#endif
 
     // ---- stage 2: gathers (clamped to particle i where not needed)
     double4 pp[kPerLane];
