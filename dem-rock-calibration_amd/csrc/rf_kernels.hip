@@ -220,7 +220,10 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
 #pragma unroll
     for (int t = 0; t < kPerLane; ++t) {
         const int k = l + kLanes * t;
-        jk[t] = s.nbr[ic * S + (k < S ? k : S - 1)];
+        if (t < kPerLane - 1 || !RF_CAND_TAIL_LATE)
+            jk[t] = s.nbr[ic * S + (k < S ? k : S - 1)];
+        else
+            jk[t] = (int32_t)ic;
         if (t == 0 || !RF_CAND_TAIL_LATE) {
             const int64_t qc = ic * C + (k < C ? k : C - 1);
             cj[t] = s.cand[qc];
@@ -239,6 +242,13 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const int32_t bp1 = s.bpart[qb1];
 
 #if RF_CAND_TAIL_LATE
+    // Slots 24..31 hold partners only in particles with many partners or a
+    // fragmented row (a few per cent of waves); the others skip their loads.
+    const bool hi_slots = __any((occ_in >> (kLanes * (kPerLane - 1))) != 0u);
+    if (hi_slots) {  // wave-uniform
+        const int k = l + kLanes * (kPerLane - 1);
+        jk[kPerLane - 1] = s.nbr[ic * S + (k < S ? k : S - 1)];
+    }
     if (__any(nc > kLanes)) {  // wave-uniform; rare
 #pragma unroll
         for (int t = 1; t < kPerLane; ++t) {
@@ -257,7 +267,12 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     for (int t = 0; t < kPerLane; ++t) {
         const int k = l + kLanes * t;
         const bool had = k < S && ((occ_in >> k) & 1u);
-        pp[t] = pos[had ? jk[t] : (int32_t)ic];
+#if RF_CAND_TAIL_LATE
+        if (t == kPerLane - 1 && !hi_slots)
+            pp[t] = pi;  // no partner in these slots anywhere in the wave
+        else
+#endif
+            pp[t] = pos[had ? jk[t] : (int32_t)ic];
         cab |= (uint32_t)((k < nc) & (k < C) & (cs[t] < 0)) << t;
     }
     // Lists are rebuilt without the current partners, so the candidates to

@@ -334,3 +334,28 @@ def test_wide_skin_many_candidates_parity():
     for chunk in (1, 30, 120):
         assert g.step(chunk) == o.step(chunk) == chunk
         compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"wide skin +{chunk}")
+
+
+def test_high_slots_parity():
+    """A neighbour cutoff of 3.45 mean radii: about one particle in ten
+    holds partners in slots 24..31 (up to 29 partners), so waves with and
+    without such particles both run (the step kernel skips those slots'
+    loads in waves where no particle uses them).  Build and 400 loading
+    steps equal the oracle's bit for bit."""
+    print("case high slots n=1500 acut 3.45 r_mean", flush=True)
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="uniaxial", seed=33,
+                             mat_kwargs=dict(acut_factor=3.45),
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=5, max_strain=0.05))
+    g, o = _pair(w)
+    g.build_bonds()
+    o.build_bonds()
+    nbr = np.asarray(o.get_state()["nbr"]).reshape(o.n, -1)
+    hi = (nbr[:, 24:] >= 0).any(axis=1)
+    assert 50 < hi.sum() < o.n // 2
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="high slots, build")
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    assert g.step(400) == o.step(400) == 400
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="high slots, 400 steps")
+    compare_test_states(g.test_state(), o.test_state(), what="high slots, 400 steps")
+    np.testing.assert_array_equal(g.curve(), o.curve())
