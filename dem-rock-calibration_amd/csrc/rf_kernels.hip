@@ -350,7 +350,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         prn_l |= (uint32_t)(had & far) << k;
         pc_l |= (uint32_t)(had & near) << k;
     }
-    const uint32_t prune = grp_or(prn_l);
+    uint32_t prune = 0u;
+    if (__any(prn_l != 0u)) prune = grp_or(prn_l);  // wave-uniform; pruning is rare
     uint32_t occ = occ_in & ~prune;
     uint32_t bnd = bnd_in & ~prune;
     otnz &= ~prune;
@@ -483,7 +484,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         }
         occ |= added;
     }
-    const uint32_t pcm = grp_or(pc_l);
+    uint32_t pcm = 0u;
+    if (__any(pc_l != 0u)) pcm = grp_or(pc_l);  // wave-uniform
 
     // ---- 2a. bonds: lane l owns bonds l + 8t (d_CalculateForcesDEM_22)
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -565,7 +567,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     }
     const double Fbx = tree8(acc[0]), Fby = tree8(acc[1]), Fbz = tree8(acc[2]);
     const double Mbx = tree8(acc[3]), Mby = tree8(acc[4]), Mbz = tree8(acc[5]);
-    bnd &= ~grp_or(brk_l);
+    if (__any(brk_l != 0u)) bnd &= ~grp_or(brk_l);  // wave-uniform; breaks are rare
 
     // ---- 2b. contacts: lane l owns slots l + 8t (d_CalculateForcesDEM_21)
     const double wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
@@ -606,9 +608,16 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
             ovl_l |= 1u << k;
         }
     }
-    otnz = (otnz & ~cmask) | grp_or(ovl_l);
-    const double Fcx = tree8(acc[0]), Fcy = tree8(acc[1]), Fcz = tree8(acc[2]);
-    double Mcx = tree8(acc[3]), Mcy = tree8(acc[4]), Mcz = tree8(acc[5]);
+    // A wave without a touching pair sums nothing but +0.0: its sums are
+    // +0.0 without the reductions.
+    double Fcx = 0.0, Fcy = 0.0, Fcz = 0.0, Mcx = 0.0, Mcy = 0.0, Mcz = 0.0;
+    if (__any(ovl_l != 0u)) {  // wave-uniform
+        otnz = (otnz & ~cmask) | grp_or(ovl_l);
+        Fcx = tree8(acc[0]); Fcy = tree8(acc[1]); Fcz = tree8(acc[2]);
+        Mcx = tree8(acc[3]); Mcy = tree8(acc[4]); Mcz = tree8(acc[5]);
+    } else {
+        otnz &= ~cmask;
+    }
     if ((p.quirks & RF_QUIRK_ROLL_RESET) && !roll) {
         Mcx = 0.0; Mcy = 0.0; Mcz = 0.0;
     }
