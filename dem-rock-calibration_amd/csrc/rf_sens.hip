@@ -279,7 +279,12 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
             }
         };
         D csum[6];
-        tree_sum(contact_lane, csum);
+        if (ovl != 0u) {
+            tree_sum(contact_lane, csum);
+        } else {  // no touching pair: the sums of +0.0 are +0.0
+#pragma unroll
+            for (int u = 0; u < 6; ++u) csum[u] = zero;
+        }
         otnz = (otnz & ~cmask) | ovl;
         if ((p.quirks & RF_QUIRK_ROLL_RESET) && !roll)
 #pragma unroll
