@@ -112,10 +112,14 @@ struct rf_ctx {
     // Captured chunks by length: the loop shortens its chunks while list
     // rebuilds come often (a rebuild halts the rest of a chunk, whose
     // launches then do nothing) and lengthens them again after full chunks.
+    // A graph bakes in the kernels' by-value arguments: the constants and
+    // every device pointer, so it is re-captured when either changes (a
+    // reallocated curve buffer, for one).
     struct Graph {
         int64_t steps = 0;
         hipGraphExec_t exec = nullptr;
         rf::DevParams p{};
+        rf::DevState d{};
     };
     std::vector<Graph> graphs;
     int32_t chunk_eff = 256;
@@ -419,10 +423,26 @@ rf_status device_error(rf_ctx* c) {
     return fail(RF_ERR_STATE, "device error flags 0x%x", e);
 }
 
+// Ends a capture begun on `st` whatever happened inside it and instantiates
+// the graph; `launch_err` is the first error a launch inside it returned.
+rf_status end_capture(hipStream_t st, hipError_t launch_err, hipGraphExec_t* exec) {
+    *exec = nullptr;
+    hipGraph_t gr = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &gr);
+    hipError_t e = launch_err != hipSuccess ? launch_err : ee;
+    if (e == hipSuccess) e = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
+    if (gr) hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return fail(RF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    return RF_OK;
+}
+
 // Launch `k` (step, controller) pairs as one captured graph.  The kernels
 // read the step's buffers and halt flag from the device control block, so the
-// same graph serves every chunk of its length; it is captured again only when
-// the constants passed by value (DevParams) change.
+// same graph serves every chunk of its length; it is captured again when the
+// arguments passed by value (DevParams, and the DevState pointers) change.
 rf_status ensure_graph(rf_ctx* c, int64_t k, rf_ctx::Graph** out) {
     rf_ctx::Graph* g = nullptr;
     for (auto& q : c->graphs)
@@ -432,25 +452,23 @@ rf_status ensure_graph(rf_ctx* c, int64_t k, rf_ctx::Graph** out) {
         g = &c->graphs.back();
         g->steps = k;
     }
-    if (!g->exec || std::memcmp(&g->p, &c->p, sizeof(rf::DevParams)) != 0) {
+    if (!g->exec || std::memcmp(&g->p, &c->p, sizeof(rf::DevParams)) != 0 ||
+        std::memcmp(&g->d, &c->d, sizeof(rf::DevState)) != 0) {
         if (g->exec) {
             HIP_TRY(hipGraphExecDestroy(g->exec));
             g->exec = nullptr;
         }
-        hipGraph_t gr = nullptr;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
         for (int64_t s = 0; s < k; ++s) {
-            rf::launch_step(c->d, c->p, c->ctrl, c->stream);
-            rf::launch_controller(c->d, c->p, c->ctrl, c->stream);
+            const hipError_t e1 = rf::launch_step(c->d, c->p, c->ctrl, c->stream);
+            const hipError_t e2 = rf::launch_controller(c->d, c->p, c->ctrl, c->stream);
+            if (le == hipSuccess) le = e1 != hipSuccess ? e1 : e2;
         }
-        HIP_TRY(hipStreamEndCapture(c->stream, &gr));
-        const hipError_t e = hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0);
-        hipGraphDestroy(gr);
-        if (e != hipSuccess) {
-            g->exec = nullptr;
-            return fail(RF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
-        }
+        rf_status st = end_capture(c->stream, le, &g->exec);
+        if (st) return st;
         g->p = c->p;
+        g->d = c->d;
     }
     *out = g;
     return RF_OK;
@@ -1288,17 +1306,17 @@ rf_status batch_launch(rf_batch* b, int64_t k, bool graphed) {
     for (auto& q : b->graphs)
         if (q.steps == k) g = &q;
     if (!g) {
-        hipGraph_t gr = nullptr;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
         for (int64_t s = 0; s < k; ++s) {
-            rf::launch_step_batch(b->d_e, b->d_map, b->d_loc, b->nblk_total, b->stream);
-            rf::launch_controller_batch(b->d_e, count, b->stream);
+            const hipError_t e1 =
+                rf::launch_step_batch(b->d_e, b->d_map, b->d_loc, b->nblk_total, b->stream);
+            const hipError_t e2 = rf::launch_controller_batch(b->d_e, count, b->stream);
+            if (le == hipSuccess) le = e1 != hipSuccess ? e1 : e2;
         }
-        HIP_TRY(hipStreamEndCapture(b->stream, &gr));
         hipGraphExec_t ex = nullptr;
-        const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-        hipGraphDestroy(gr);
-        if (e != hipSuccess) return fail(RF_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        rf_status st = end_capture(b->stream, le, &ex);
+        if (st) return st;
         b->graphs.push_back(rf_batch::Graph{k, ex});
         g = &b->graphs.back();
     }

@@ -166,16 +166,26 @@ WORKLOADS = {
 
 def make_workload(name=None, n=None, r_min=0.2, r_max=0.3, shape="cylinder", aspect=2.0,
                   test="uniaxial", seed=20261203, gen_kwargs=None, mat_kwargs=None,
-                  test_kwargs=None):
-    """Generate (or fetch) a sample and build everything a test run needs."""
+                  test_kwargs=None, generator="fcc"):
+    """Generate a sample and build everything a test run needs.
+
+    generator "rsi": random sequential insertion then FIRE on the GPU
+    (sample.generate_rsi), the loading configurations' generator; "fcc":
+    the CPU jittered-lattice generator (sample.generate)."""
     if name is not None:
         w = dict(WORKLOADS[name])
         n = n if n is not None else w["n"]
         r_min, r_max, shape, aspect, test = (w["r_min"], w["r_max"], w["shape"], w["aspect"],
                                              w["test"])
-    gk = dict(packing=0.6, tol=0.02, jitter=0.05, max_iter=300)
-    gk.update(gen_kwargs or {})
-    xyz, r = sample.generate(n, r_min, r_max, shape=shape, aspect=aspect, seed=seed, **gk)
+    if generator == "rsi":
+        gk = dict(packing=0.6)
+        gk.update(gen_kwargs or {})
+        xyz, r, _ = sample.generate_rsi(n, r_min, r_max, shape=shape, aspect=aspect, seed=seed,
+                                        **gk)
+    else:
+        gk = dict(packing=0.6, tol=0.02, jitter=0.05, max_iter=300)
+        gk.update(gen_kwargs or {})
+        xyz, r = sample.generate(n, r_min, r_max, shape=shape, aspect=aspect, seed=seed, **gk)
     mat = material(r_min, 0.5 * (r_min + r_max), **(mat_kwargs or {}))
     factor = (1.6, 1.6, 1.2) if shape == "cylinder" else (1.3, 1.3, 1.5)
     xyz, box = place_in_box(xyz, r, mat, factor)
@@ -187,3 +197,72 @@ def make_workload(name=None, n=None, r_min=0.2, r_max=0.3, shape="cylinder", asp
     else:
         cfg = None
     return dict(xyz=xyz, r=r, mat=mat, box=box, cfg=cfg, n=n, shape=shape)
+
+
+# ------------------------------------------------- the loading configurations
+# C1-C3: one loading test each; C4: a batch of samples; C5: the elastic
+# pre-peak window the gradients are taken over.  Every sample comes from the
+# random-sequential-insertion + FIRE generator (sample.generate_rsi, GPU).
+STRAIN_RATE = 0.1        # 1/s, uniaxial configurations
+BRAZILIAN_SPEED = 0.02   # m/s, platens of the Brazilian configuration
+SEED = 20261203
+
+
+def config_workload(name, n=None, seed=None, dnsave=100, **test_kwargs):
+    """C1 (5k uniaxial), C2 (100k uniaxial) or C3 (50k Brazilian disc) with
+    the configuration's loading: platens closing at strain rate 0.1/s times
+    the specimen height (uniaxial) or at 0.02 m/s (Brazilian)."""
+    idx = {"C1": 1, "C2": 2, "C3": 3}[name]
+    w = make_workload(name, n=n, seed=SEED + idx if seed is None else seed, generator="rsi")
+    if WORKLOADS[name]["test"] == "uniaxial":
+        v = -STRAIN_RATE * w["cfg"].size[2] * 1e-3
+        w["cfg"] = uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v, dnsave=dnsave, **test_kwargs)
+    else:
+        w["cfg"] = brazilian_cfg(w["xyz"], w["r"], v_load_mps=-BRAZILIAN_SPEED, dnsave=dnsave,
+                                 **test_kwargs)
+    w["name"] = name
+    return w
+
+
+def batch_workloads(count=64, seed=SEED + 4, n_lo=2000, n_hi=40000):
+    """C4: `count` uniaxial samples, N log-uniform in [n_lo, n_hi], radii
+    U[0.20, 0.30] mm, micro-parameters on a Latin hypercube: bond E in
+    [5, 50] GPa, normal strength in [10, 100] MPa (shear strength 1.5 times
+    it), friction in [0.3, 0.8]; loaded at strain rate 0.1/s."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), count)).astype(int)
+
+    def lhs(lo, hi):
+        u = (rng.permutation(count) + rng.uniform(0.0, 1.0, count)) / count
+        return lo + (hi - lo) * u
+
+    E, sig, mu = lhs(5.0, 50.0), lhs(10.0, 100.0), lhs(0.3, 0.8)
+    out = []
+    for k in range(count):
+        w = make_workload(n=int(ns[k]), r_min=0.20, r_max=0.30, test="uniaxial",
+                          seed=seed + 100 + k, generator="rsi",
+                          mat_kwargs=dict(E_bond_gpa=float(E[k]), sigma_mpa=float(sig[k]),
+                                          tau_mpa=float(1.5 * sig[k]), friction=float(mu[k])))
+        v = -STRAIN_RATE * w["cfg"].size[2] * 1e-3
+        w["cfg"] = uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v, dnsave=100)
+        w["lhs"] = dict(E_bond_gpa=float(E[k]), sigma_mpa=float(sig[k]), friction=float(mu[k]))
+        out.append(w)
+    return out
+
+
+def gradient_workload(n=20000, steps=2000, seed=SEED + 5, dnsave=50):
+    """C5: a 20k-sphere cylinder (r ~ U[0.20, 0.30] mm) loaded for `steps`
+    elastic pre-peak steps to 0.1 % axial strain (closing speed
+    0.001 H / (steps dt)), with dt fixed at the value rf_derive gives (the
+    parameters the gradients are taken for do not move it)."""
+    from . import core
+
+    w = make_workload(n=n, r_min=0.20, r_max=0.30, test="uniaxial", seed=seed, generator="rsi",
+                      test_kwargs=dict(dnsave=dnsave, max_steps=100 * steps, max_strain=0.05,
+                                       min_stress_mpa=0.1))
+    m = abi.Material.from_buffer_copy(w["mat"])
+    m.dt = core.derive(w["mat"], float(np.min(w["r"])), float(np.max(w["r"]))).dt
+    w["mat"] = m
+    w["cfg"].v_load = -0.001 * w["cfg"].size[2] / (steps * m.dt)
+    w["steps"] = steps
+    return w

@@ -1,20 +1,27 @@
-"""Seeded sample generator and the reference's Sample_*.txt format.
+"""Seeded sample generators and the reference's Sample_*.txt format.
 
 The reference builds its specimens with a time-seeded cuRAND fill followed by
 compaction and FIRE (md_createcylindersample.cu:184-439), so its samples
 cannot be reproduced.  This module writes its own seeded, polydisperse
-specimens instead:
+specimens instead, in one of two ways:
 
-1. radii are drawn uniform in [r_min, r_max];
-2. centres start on a jittered face-centred-cubic lattice sized so that the
-   solid fraction inside the cylinder (or disc) equals ``packing``;
-3. overlaps are relaxed by damped Jacobi pushes (every overlapping pair is
-   pushed apart along its normal, the container keeps every sphere inside)
-   until the largest overlap is below ``tol`` times r_min.
+``generate_rsi`` (the loading configurations: random sequential insertion
+then FIRE): radii uniform in [r_min, r_max]; spheres inserted one after
+another at random centres in the container at 0.6 of their radius, without
+overlaps; then grown to full size in stages, each relaxed by FIRE on the GPU
+(rf_fire_relax).  Particles come in insertion order, i.e. in no spatial
+order.
 
-The result is a dense random packing with many near-touching pairs, which is
-what the bond criterion (gap < bond_gap) needs.  It is deterministic for a
-given seed on a given machine (numpy PCG64 plus ordered pair sets).
+``generate`` (CPU only, for tests that need no GPU): centres on a jittered
+face-centred-cubic lattice sized so that the solid fraction inside the
+cylinder (or disc) equals ``packing``, then overlaps relaxed by damped Jacobi
+pushes (every overlapping pair pushed apart along its normal, the container
+keeping every sphere inside) until the largest overlap is below ``tol``
+times r_min.
+
+Either gives a dense random packing with many near-touching pairs, which is
+what the bond criterion (gap < bond_gap) needs, deterministic for a given
+seed on a given machine.
 
 File format (md_createcylindersample.cu:144-182): first line N, then one
 ``x y z`` line per sphere in metres.  A fourth column, the radius in metres,
@@ -139,6 +146,113 @@ def generate(n, r_min, r_max, shape="cylinder", aspect=2.0, packing=0.60, seed=2
                           - np.bincount(a_i, weights=push[:, k], minlength=n))
         _container_push(pts, r, R, H)
     return pts, r
+
+
+RSI_STAGES = (0.6, 0.75, 0.85, 0.92, 0.97, 1.0)
+
+
+def _rsi(r, Rc, Hc, rng, batch=4096, tries=16):
+    """Random sequential insertion: spheres are inserted in index order, in
+    batches, each at uniform random centres in the cylinder of radius Rc and
+    half height Hc; a sphere takes the first of its `tries` candidate centres
+    that overlaps no sphere already placed, and, inside its batch, no sphere
+    of lower index accepted in the same batch.  Spheres left without a place
+    try again in the next batch.  No two placed spheres overlap."""
+    from scipy.spatial import cKDTree
+
+    n = len(r)
+    rmax = float(r.max())
+    pos = np.zeros((n, 3))
+    done = np.zeros(n, dtype=bool)
+    pending = np.arange(n)
+    tree, tree_idx = None, None
+    rounds = 0
+    while pending.size:
+        rounds += 1
+        if rounds > 100 * (n // batch + 1):
+            raise RuntimeError(f"random sequential insertion placed {int(done.sum())} of {n}")
+        idx = pending[:batch]
+        m = len(idx)
+        ang = rng.uniform(0.0, 2.0 * np.pi, (m, tries))
+        rad = Rc * np.sqrt(rng.uniform(0.0, 1.0, (m, tries)))
+        cand = np.stack([rad * np.cos(ang), rad * np.sin(ang), rng.uniform(-Hc, Hc, (m, tries))],
+                        axis=2)
+        ok = np.ones((m, tries), dtype=bool)
+        if tree is not None:
+            dist, nb = tree.query(cand.reshape(-1, 3), k=24, distance_upper_bound=2.0 * rmax)
+            have = nb < len(tree_idx)
+            rj = np.where(have, r[tree_idx[np.minimum(nb, len(tree_idx) - 1)]], 0.0)
+            ri = np.repeat(r[idx], tries)[:, None]
+            ok = ~np.any(have & (dist < ri + rj), axis=1).reshape(m, tries)
+        has = ok.any(axis=1)
+        first = np.argmax(ok, axis=1)
+        prop = cand[np.arange(m), first]
+        acc = has.copy()
+        sel = np.nonzero(has)[0]
+        if sel.size > 1:
+            pairs = cKDTree(prop[sel]).query_pairs(2.0 * rmax, output_type="ndarray")
+            if len(pairs):
+                a_, b_ = sel[pairs[:, 0]], sel[pairs[:, 1]]
+                lo, hi = np.minimum(a_, b_), np.maximum(a_, b_)
+                d = np.linalg.norm(prop[lo] - prop[hi], axis=1)
+                clash = d < r[idx[lo]] + r[idx[hi]]
+                lo, hi = lo[clash], hi[clash]
+                for q in np.argsort(hi, kind="stable"):  # later sphere of each pair, in order
+                    if acc[lo[q]] and acc[hi[q]]:
+                        acc[hi[q]] = False
+        put = idx[acc]
+        pos[put] = prop[acc]
+        done[put] = True
+        pending = np.concatenate([idx[~acc], pending[batch:]])
+        tree_idx = np.nonzero(done)[0]
+        tree = cKDTree(pos[tree_idx])
+    return pos
+
+
+def generate_rsi(n, r_min, r_max, shape="cylinder", aspect=2.0, packing=0.60, seed=20261203,
+                 stages=RSI_STAGES, gap_tol=-0.005, device=0, verbose=False):
+    """Seeded random sequential insertion then FIRE relaxation, on the GPU.
+
+    Radii are uniform in [r_min, r_max].  The container (axis z) holds the
+    sphere centres in a cylinder of radius D/2 - r_mean and half height
+    H/2 - r_mean, with D and H (= aspect D; for a disc, aspect is the
+    thickness over the diameter) sized so that the spheres fill `packing` of
+    the D x H cylinder.  Spheres are inserted one after another at
+    stages[0] times their radius (no overlaps), then grown through the
+    stages; each stage is relaxed by FIRE (rf_fire_relax: harmonic overlap
+    force, absorbing cylinder walls) until no pair overlaps by more than
+    -gap_tol of its radius sum (last stage) or 1e-3 (earlier stages).  The
+    reference fills a box at random with a time-seeded generator and relaxes
+    it the same way (md_createcylindersample.cu:184-439).  Returns
+    (xyz[n, 3], r[n]) in mm in insertion order, centred at the origin, and a
+    dict of the stages' FIRE results."""
+    from . import abi, fire
+
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(r_min, r_max, n)
+    rm = float(r.mean())
+    vc = float(np.sum(4.0 / 3.0 * np.pi * r ** 3)) / packing
+    D = (4.0 * vc / (np.pi * aspect)) ** (1.0 / 3.0)
+    H = aspect * D
+    Rc, Hc = 0.5 * D - rm, 0.5 * H - rm
+    s0 = stages[0]
+    pos = _rsi(r * s0, Rc, Hc, rng)
+    info = []
+    for q, sc in enumerate(stages):
+        last = q == len(stages) - 1
+        c = fire.defaults(D=1.0, dt0=0.005, density=2.65, quirks=0, max_steps=50,
+                          hard_cap=60000 if last else 20000,
+                          gap_tol=gap_tol if last else min(gap_tol, -1e-3),
+                          check_every=50 if last else 20, delete_after=-1,
+                          container=abi.RF_CONTAINER_CYLINDER_Z_ABSORB, center=(0.0, 0.0, 0.0),
+                          radius=Rc, half_height=Hc, hide_z=1e30, hidden=(0.0, 0.0, 1e6),
+                          law=abi.RF_FIRE_LAW_HARMONIC, device=device)
+        pos, _, res = fire.relax(pos, r * sc, c)
+        info.append(dict(scale=sc, steps=res.steps, min_gap=res.min_gap, stop=res.stop_reason))
+        if verbose:
+            print(f"rsi stage {sc}: {res.steps} FIRE steps, min gap {res.min_gap:.3e}, "
+                  f"stop {res.stop_reason}", flush=True)
+    return pos, r, info
 
 
 def cylinder_geometry(xyz, r):

@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_dem.so")
+# RF_ORACLE_LIB loads another build of the oracle (the mutation test of
+# tests/test_oracle_mutants.py points it at deliberately broken copies).
+LIB_PATH = os.environ.get("RF_ORACLE_LIB") or os.path.join(_HERE, "liboracle_dem.so")
 
 
 def build(force=False):

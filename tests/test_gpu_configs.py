@@ -1,0 +1,114 @@
+"""The loading configurations through the HIP path, against the CPU oracle.
+
+C1: 5k-sphere cylinder, strain rate 0.1/s, 2000 loading steps.
+C3: 50k-sphere Brazilian disc, platens at 0.02 m/s, 500 steps.
+C4: 64 samples (N log-uniform in [2k, 40k], Latin-hypercube material) as one
+    batch for 200 steps: every sample equal to its own single run, two of
+    them equal to the oracle.
+C5: the 20k-sphere elastic window of the gradients (2000 steps to 0.1 %
+    strain, dt fixed), forward run.
+All samples come from the random-sequential-insertion + FIRE generator.
+Bar: lists, bond numbering and breaks identical; state, histories,
+controller, curve and bond counts bitwise equal.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from helpers import compare_states, compare_test_states
+from rockfit import config
+from rockfit.core import Batch, Context
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16
+
+
+def _run_pair(w, chunks, what):
+    g = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], threads=THREADS)
+    g.build_bonds()
+    o.build_bonds()
+    compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"{what} build")
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    done = 0
+    for chunk in chunks:
+        kg, ko = g.step(chunk), o.step(chunk)
+        assert kg == ko == chunk, (kg, ko)
+        done += chunk
+        print(f"{what}: {done} steps equal", flush=True)
+        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"{what} step {done}")
+        compare_test_states(g.test_state(), o.test_state(), what=f"{what} step {done}")
+    np.testing.assert_array_equal(g.curve(), o.curve())
+    np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
+    return g, o
+
+
+@pytest.mark.timeout(300)
+def test_c1_uniaxial_5k_strain_rate():
+    print("case C1 5k uniaxial 0.1/s", flush=True)
+    w = config.config_workload("C1")
+    g, _ = _run_pair(w, (1, 499, 1500), "C1")
+    ts = g.test_state()
+    assert ts.step == 2000 and ts.stress > 0.0  # the platens load the specimen
+    c = g.counters()
+    assert c.bonds > 4 * c.n
+
+
+@pytest.mark.timeout(300)
+def test_c3_brazilian_50k():
+    print("case C3 50k Brazilian 0.02 m/s", flush=True)
+    w = config.config_workload("C3")
+    g, _ = _run_pair(w, (1, 99, 400), "C3")
+    assert g.test_state().step == 500
+
+
+@pytest.mark.timeout(600)
+def test_c4_batch_64_samples():
+    print("case C4 64-sample batch", flush=True)
+    ws = config.batch_workloads(64)
+    steps = 200
+
+    def make(w):
+        c = Context(w["xyz"], w["r"], w["mat"], w["box"])
+        c.build_bonds()
+        c.set_test(w["cfg"])
+        return c
+
+    bat = [make(w) for w in ws]
+    b = Batch(bat)
+    done = b.step(steps)
+    assert all(d == steps for d in done), done
+    print(f"C4: batch of {len(ws)} samples, {sum(w['n'] for w in ws)} spheres stepped", flush=True)
+    singles = []
+    for k, w in enumerate(ws):
+        s = make(w)
+        assert s.step(steps) == steps
+        compare_states(bat[k].get_state(), s.get_state(), s.n, s.slots,
+                       what=f"C4 sample {k} batch vs single")
+        compare_test_states(bat[k].test_state(), s.test_state(), what=f"C4 sample {k}")
+        np.testing.assert_array_equal(bat[k].curve(), s.curve())
+        singles.append(s)
+    print("C4: every sample equals its single run", flush=True)
+    order = np.argsort([w["n"] for w in ws])
+    for k in (int(order[0]), int(order[len(order) // 2])):
+        w = ws[k]
+        o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], threads=THREADS)
+        o.build_bonds()
+        o.set_test(w["cfg"])
+        assert o.step(steps) == steps
+        compare_states(bat[k].get_state(), o.get_state(), w["n"], 32,
+                       what=f"C4 sample {k} (n={w['n']}) vs oracle")
+        compare_test_states(bat[k].test_state(), o.test_state(), what=f"C4 sample {k} oracle")
+        np.testing.assert_array_equal(bat[k].curve(), o.curve())
+        print(f"C4: sample {k} (n={w['n']}) equals the oracle", flush=True)
+
+
+@pytest.mark.timeout(300)
+def test_c5_gradient_window_forward():
+    print("case C5 20k 2000-step window forward", flush=True)
+    w = config.gradient_workload()
+    g, _ = _run_pair(w, (1000, 1000), "C5")
+    ts = g.test_state()
+    assert ts.step == 2000 and ts.stress > 0.0

@@ -56,3 +56,20 @@ def test_bond_strength_draw_statistics():
     assert np.log(vals).std() == pytest.approx(np.sqrt(np.log(1.04)), rel=0.03)
     assert L.ora_bond_strength(99, 3, 10, 0, 40.0, 0.2) == L.ora_bond_strength(99, 3, 10, 0, 40.0, 0.2)
     assert L.ora_bond_strength(99, 3, 10, 0, 40.0, 0.0) == 40.0
+
+
+def test_random_sequential_insertion():
+    """RSI places every sphere inside the container without overlaps, in a
+    seeded, repeatable way."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(3)
+    n = 6000
+    r = rng.uniform(0.12, 0.18, n)
+    Rc, Hc = 3.0, 6.0
+    p = sample._rsi(r, Rc, Hc, np.random.default_rng(9))
+    assert np.all(np.hypot(p[:, 0], p[:, 1]) <= Rc) and np.all(np.abs(p[:, 2]) <= Hc)
+    pairs = cKDTree(p).query_pairs(2 * r.max(), output_type="ndarray")
+    d = np.linalg.norm(p[pairs[:, 0]] - p[pairs[:, 1]], axis=1)
+    assert np.all(d >= r[pairs[:, 0]] + r[pairs[:, 1]])
+    np.testing.assert_array_equal(p, sample._rsi(r, Rc, Hc, np.random.default_rng(9)))
