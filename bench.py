@@ -55,9 +55,9 @@ def step_kernel_bytes(c):
     return 144 * c.n + 5 * c.occupied + 200 * c.bonds + 48 * c.contacts
 
 
-def measured_traffic(n):
+def measured_traffic(whash):
     """HBM bytes per step-kernel launch from the committed PMC summary, when
-    it was measured on this particle count; else None."""
+    it was measured on this very workload (same hash); else None."""
     try:
         with open(PMC_SUMMARY) as f:
             lines = f.read().splitlines()
@@ -65,15 +65,27 @@ def measured_traffic(n):
         return None
     vals = {}
     for ln in lines:
-        if ":" in ln and ln.split(":")[0].strip() in ("n_particles", "traffic_bytes_per_launch"):
+        if ":" in ln and ln.split(":")[0].strip() in ("workload_hash", "traffic_bytes_per_launch"):
             k, v = ln.split(":", 1)
-            vals[k.strip()] = float(v.strip().strip("`"))
-    if vals.get("n_particles") == n and "traffic_bytes_per_launch" in vals:
-        return vals["traffic_bytes_per_launch"]
+            vals[k.strip()] = v.strip().strip("`")
+    if vals.get("workload_hash") == whash and "traffic_bytes_per_launch" in vals:
+        return float(vals["traffic_bytes_per_launch"])
     return None
 
 
-def cpu_baseline(w, steps, threads):
+def host_cores():
+    """Cores this job may use: OMP_NUM_THREADS when set (the GPU hosts set it
+    to the job's CPU share), else the process's CPU affinity."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_leg(w, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -85,9 +97,24 @@ def cpu_baseline(w, steps, threads):
     t0 = time.perf_counter()
     k = o.step(steps)
     dt = time.perf_counter() - t0
-    return dict(value=w["n"] * k / dt, unit="particle-steps/s", cores=threads, kind="port",
-                sample=f"{k} loading steps of the same {w['n']}-particle workload "
-                       f"after 2 warm-up steps, FP64 CPU oracle, OpenMP {threads} threads")
+    return w["n"] * k / dt, k
+
+
+def cpu_baseline(w, threads, seconds=12.0):
+    """The FP64 CPU oracle on the same workload, on all the job's cores and on
+    one thread, each for about `seconds` of CPU work."""
+    n = w["n"]
+    k_all = max(5, int(seconds * 6e5 * threads / n))
+    log(f"cpu baseline: {k_all} steps on {threads} threads")
+    v_all, k_all = cpu_leg(w, k_all, threads)
+    k_one = max(3, int(seconds * 7e5 / n))
+    log(f"cpu baseline: {k_one} steps on 1 thread")
+    v_one, k_one = cpu_leg(w, k_one, 1)
+    return dict(value=v_all, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"{k_all} loading steps of the same {n}-particle workload after 2 "
+                       f"warm-up steps, FP64 CPU oracle, OpenMP {threads} threads",
+                single_thread={"value": v_one, "cores": 1,
+                               "sample": f"{k_one} loading steps, 1 thread"})
 
 
 def batch_bench(count, steps, warmup):
@@ -102,25 +129,8 @@ def batch_bench(count, steps, warmup):
     from rockfit import config
     from rockfit.core import Batch, Context
 
-    rng = np.random.default_rng(20261203 + 4)
-    ns = np.exp(rng.uniform(np.log(2000.0), np.log(40000.0), count)).astype(int)
-
-    def lhs(lo, hi):
-        u = (rng.permutation(count) + rng.uniform(0.0, 1.0, count)) / count
-        return lo + (hi - lo) * u
-
-    E, sig, mu = lhs(5.0, 50.0), lhs(10.0, 100.0), lhs(0.3, 0.8)
     t0 = time.perf_counter()
-    ws = []
-    for k in range(count):
-        w = config.make_workload(n=int(ns[k]), r_min=0.20, r_max=0.30, test="uniaxial",
-                                 seed=20261203 + 100 + k, gen_kwargs=dict(relax="gpu"),
-                                 mat_kwargs=dict(E_bond_gpa=float(E[k]), sigma_mpa=float(sig[k]),
-                                                 tau_mpa=float(1.5 * sig[k]),
-                                                 friction=float(mu[k])))
-        h = w["cfg"].size[2]
-        w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=-0.1 * h * 1e-3, dnsave=100)
-        ws.append(w)
+    ws = config.batch_workloads(count)
     total_n = int(sum(w["n"] for w in ws))
     log(f"{count} samples, {total_n} particles, generated in {time.perf_counter() - t0:.1f}s")
 
@@ -167,7 +177,7 @@ def batch_bench(count, steps, warmup):
         "metric": f"particle-steps/s, {count}-sample batch (N log-U[2k,40k]) on 1 GPU",
         "value": pst_bat, "unit": "particle-steps/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "higher_is_better": True, "dtype": "f64",
-        "data": "synthetic (seeded generator, FIRE-relaxed on the GPU)",
+        "data": "synthetic (seeded random sequential insertion + FIRE on the GPU)",
         "config": {"workload": "batch of uniaxial tests", "samples": count,
                    "particles": total_n},
         "sequential": {"value": pst_seq, "unit": "particle-steps/s",
@@ -200,13 +210,9 @@ def grad_bench(steps=2000, n=20000):
     from rockfit import autograd, config
     from rockfit.core import Context
 
-    w = config.make_workload(n=n, r_min=0.20, r_max=0.30, test="uniaxial", seed=5,
-                             gen_kwargs=dict(relax="gpu"),
-                             test_kwargs=dict(dnsave=50, max_steps=100000, max_strain=0.05,
-                                              min_stress_mpa=0.1))
-    mat = autograd.fixed_dt(w["mat"], w["r"])
+    w = config.gradient_workload(n=n, steps=steps)
+    mat = w["mat"]
     cfg = w["cfg"]
-    cfg.v_load = -0.001 * cfg.size[2] / (steps * mat.dt)
     names = ["E_bond", "bond_radius_ratio", "E_contact"]
     theta = np.array([getattr(mat, autograd.PARAMS[k][1]) for k in names])
 
@@ -275,7 +281,7 @@ def grad_bench(steps=2000, n=20000):
         "steps": steps,
         "higher_is_better": False,
         "dtype": "f64",
-        "data": "synthetic (seeded generator, FIRE-relaxed on the GPU)",
+        "data": "synthetic (seeded random sequential insertion + FIRE on the GPU)",
         "config": {"workload": f"BPM uniaxial pre-peak window, {n} spheres r~U[0.20,0.30] mm, "
                                f"{steps} steps to 0.1% strain",
                    "directions": names, "forward_ms": fwd * 1e3, "gradient_ms": grad * 1e3,
@@ -299,9 +305,12 @@ def main():
                          "0.1/s (the default line); uniaxial-5k: the same test with "
                          "r ~ U[0.20, 0.30] mm; brazilian-50k: 2:1 disc, r ~ U[0.20, 0.30] mm, "
                          "platens at 0.02 m/s")
-    ap.add_argument("--cpu-steps", type=int, default=-1,
-                    help="CPU-baseline steps (default: sized to ~15 s of CPU work)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU work of each CPU-baseline leg (all cores, one thread)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-core leg (default: the job's cores)")
+    ap.add_argument("--cache", default=None,
+                    help=".npz file holding the generated sample (written if absent)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=0,
                     help="measure a batch of this many samples against sequential runs and exit")
@@ -337,27 +346,23 @@ def main():
 
     t0 = time.perf_counter()
     wl = args.workload
-    n_default = {"uniaxial-100k": 100000, "uniaxial-5k": 5000, "brazilian-50k": 50000}[wl]
-    n = args.n or n_default
-    if wl == "brazilian-50k":
-        w = config.make_workload(n=n, r_min=0.20, r_max=0.30, shape="disc", aspect=0.5,
-                                 test="brazilian")
-        v_mps = -0.02
-        w["cfg"] = config.brazilian_cfg(w["xyz"], w["r"], v_load_mps=v_mps, dnsave=100)
+    name = {"uniaxial-100k": "C2", "uniaxial-5k": "C1", "brazilian-50k": "C3"}[wl]
+    w = config.config_workload(name, n=args.n or None, cache=args.cache)
+    n = w["n"]
+    if name == "C3":
+        v_mps = -config.BRAZILIAN_SPEED
         desc = (f"BPM Brazilian disc test, {n} spheres r~U[0.20,0.30] mm, 2:1 disc, "
                 f"platens at 0.02 m/s, FP64")
         metric = f"particle-steps/s at {n // 1000}k-particle BPM Brazilian test (1 GPU)"
     else:
-        r_max = 0.35 if wl == "uniaxial-100k" else 0.30
-        w = config.make_workload(n=n, r_min=0.20, r_max=r_max, test="uniaxial")
-        # strain rate 0.1/s: closing speed = 0.1/s * specimen height
-        v_mps = -0.1 * w["cfg"].size[2] * 1e-3
-        w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v_mps, dnsave=100)
+        r_max = config.WORKLOADS[name]["r_max"]
+        v_mps = w["cfg"].v_load * config.VELOCITY
         desc = (f"BPM uniaxial compression, {n} spheres r~U[0.20,{r_max:.2f}] mm, "
                 f"2:1 cylinder, strain rate 0.1/s, FP64")
         metric = f"particle-steps/s at {n // 1000}k-particle BPM uniaxial test (1 GPU)"
-    log(f"sample n={w['n']} generated in {time.perf_counter() - t0:.1f}s, "
-        f"v_load {v_mps:.3e} m/s")
+    whash = config.workload_hash(w)
+    log(f"sample n={w['n']} ready in {time.perf_counter() - t0:.1f}s, "
+        f"v_load {v_mps:.3e} m/s, workload hash {whash}")
 
     device_sync()
     t_setup = time.perf_counter()
@@ -374,7 +379,11 @@ def main():
     device_sync()
     log(f"warm-up {done} steps")
 
+    # The timed window runs in chunks of up to 256 steps, each replayed as a
+    # captured graph; the graph of that chunk length is captured here, before
+    # the clock starts (capturing runs no step).
     ctx.set_timing(TIMING_STRIDE)
+    ctx.capture_chunk(min(args.steps, 256))
     device_sync()
     t1 = time.perf_counter()
     done = ctx.step(args.steps)
@@ -398,12 +407,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        ksteps = args.cpu_steps
-        if ksteps < 0:
-            ksteps = max(5, int(15.0 * 5e5 * threads / c.n))
-        log(f"cpu baseline: {ksteps} steps on {threads} threads")
-        cpu = cpu_baseline(w, ksteps, threads)
+        cpu = cpu_baseline(w, args.cpu_threads or host_cores(), args.cpu_seconds)
 
     out = {
         "metric": metric,
@@ -417,14 +421,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded generator: jittered-FCC start, overlap relaxation)",
+        "data": "synthetic (seeded random sequential insertion + FIRE on the GPU, "
+                "particles in insertion order)",
         "config": {"workload": desc,
                    "n_particles": c.n, "partners_per_particle": c.occupied / c.n,
                    "bonds_per_particle": c.bonds / c.n,
                    "contacts_per_particle": c.contacts / c.n,
-                   "step_kernel_us": kern_avg_ms * 1e3, "rebuilds": c.rebuilds},
+                   "step_kernel_us": kern_avg_ms * 1e3, "rebuilds": c.rebuilds,
+                   "workload_hash": whash},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(c.n)},
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(whash)},
         "cpu_baseline": cpu,
         "host_inclusive": {
             "value": c.n * done / (t_setup + wall + t_read), "unit": "particle-steps/s",

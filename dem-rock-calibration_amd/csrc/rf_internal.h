@@ -74,6 +74,9 @@ struct DevState {
     // cell sort
     int32_t* ckey; int32_t* ccount; int32_t* cstart; int32_t* cfill; int32_t* ctmp;
     int32_t* sorted;
+    // caller's id of each stored particle (particles are stored in spatial
+    // order; ids rank new pairs and order the cells, as in the reference)
+    int32_t* orig;
     // reductions
     double* part_b; double* part_t; int32_t* part_bonds;
     // curve
@@ -100,6 +103,7 @@ struct DevParams {
     double v_load, min_stress, max_strain, cond_fracture, zero_stress, inv_area;
     int32_t dnsave, store_fm;
     int64_t max_steps, curve_cap;
+    int64_t zero_idx;    // storage index of the caller's particle 0
 };
 
 // Device control block: written by the controller kernel once per step.

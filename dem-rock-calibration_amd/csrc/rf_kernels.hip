@@ -411,7 +411,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const double rs = ri + pj.w;
 #pragma unroll
         for (int u = 0; u < kPerLane; ++u)
-            if (u == tc && nw) key[u] = (sp << 32) | (uint32_t)jc;
+            if (u == tc && nw) key[u] = (sp << 32) | (uint32_t)s.orig[jc];
         nw_l |= (uint32_t)nw << tc;
         npc |= (uint32_t)(nw & (rr < rs * rs * 1.000001)) << tc;
         crem &= crem - 1u;  // clears bit tc
@@ -449,7 +449,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
             }
         }
         const uint32_t freem = ~occ & (S >= 32 ? 0xffffffffu : ((1u << S) - 1u));
-        const bool q0 = (p.quirks & RF_QUIRK_SLOT0_SENTINEL) && i == 0;
+        const bool q0 = (p.quirks & RF_QUIRK_SLOT0_SENTINEL) && i == p.zero_idx;
         if (total > __popc(freem) && l == 0) atomicOr(&ctrl->err, kErrRowOverflow);
         uint32_t add_l = 0u, pcadd_l = 0u;
 #pragma unroll
@@ -993,16 +993,18 @@ __global__ void k_cell_scatter(DevState s, DevParams p) {
     s.ctmp[at] = (int32_t)i;
 }
 
-// Within a cell the ids are put in ascending order: the stable (cell, id)
-// order of cub::DeviceRadixSort::SortPairs over identity values
-// (md_cell_distribution.cu:87).  Rank = number of smaller ids in the cell.
+// Within a cell the particles are put in ascending order of the caller's
+// ids: the stable (cell, id) order of cub::DeviceRadixSort::SortPairs over
+// identity values (md_cell_distribution.cu:87).  Rank = number of smaller
+// ids in the cell.
 __global__ void k_cell_rank(DevState s, DevParams p) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     const int32_t key = s.ckey[i];
     const int32_t st = s.cstart[key], ct = s.ccount[key];
+    const int32_t oi = s.orig[i];
     int32_t rank = 0;
-    for (int32_t q = 0; q < ct; ++q) rank += s.ctmp[st + q] < (int32_t)i ? 1 : 0;
+    for (int32_t q = 0; q < ct; ++q) rank += s.orig[s.ctmp[st + q]] < oi ? 1 : 0;
     s.sorted[st + rank] = (int32_t)i;
 }
 

@@ -127,6 +127,9 @@ struct rf_ctx {
     double last_kernel_ms = 0.0, last_total_ms = 0.0;
     int64_t last_steps = 0, last_timed = 0;
     double ld = 0.0, destruction = 0.0;
+    // Storage order: orig[p] is the caller's id of stored particle p, inv its
+    // inverse (RF_ORDER_SPATIAL sorts by space at creation).
+    std::vector<int32_t> orig, inv;
 
     template <class T>
     rf_status alloc(T** out, size_t count) {
@@ -199,6 +202,46 @@ rf_status rf_box_reference(const double space[3], double cell, rf_box* b) {
     return RF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Morton code of three 21-bit cell coordinates: bit 3b holds bit b of x,
+// bit 3b + 1 of y, bit 3b + 2 of z.
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t k = 0;
+    for (int b = 20; b >= 0; --b)
+        k = (k << 3) | ((uint64_t)((z >> b) & 1u) << 2) | ((uint64_t)((y >> b) & 1u) << 1) |
+            (uint64_t)((x >> b) & 1u);
+    return k;
+}
+
+uint32_t quarter_cell(double x, double s) {
+    const double q = std::floor(x * s);
+    if (!(q > 0.0)) return 0u;
+    return q >= 2097151.0 ? 2097151u : (uint32_t)q;
+}
+
+// Spatial storage order: particles sorted by the Morton code of their
+// quarter-cell coordinates (floor(4 x / cell)), ties by id.
+std::vector<int32_t> spatial_order(const rf_sample* s, double cell) {
+    const int64_t n = s->n;
+    const double sc = 4.0 * (1.0 / cell);
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (int64_t i = 0; i < n; ++i)
+        key[i] = {morton3(quarter_cell(s->x[i], sc), quarter_cell(s->y[i], sc),
+                          quarter_cell(s->z[i], sc)),
+                  (int32_t)i};
+    std::sort(key.begin(), key.end());
+    std::vector<int32_t> ord(n);
+    for (int64_t p = 0; p < n; ++p) ord[p] = key[p].second;
+    return ord;
+}
+
+}  // namespace
+
+extern "C" {
+
 rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* box,
                     const rf_options* opt, rf_ctx** out) {
     if (!s || !mat || !box || !out || s->n <= 0 || !s->x || !s->y || !s->z || !s->r)
@@ -216,7 +259,11 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
         if (opt->skin >= 0.0) o.skin = opt->skin;
         o.device = opt->device;
         o.stream = opt->stream;
+        o.order = opt->order;
     }
+    if (o.order != RF_ORDER_SPATIAL && o.order != RF_ORDER_GIVEN)
+        return fail(RF_ERR_ARG, "unknown particle order %d", o.order);
+    if (s->n >= ((int64_t)1 << 31)) return fail(RF_ERR_ARG, "rf_create: n too large");
     if (o.slots > rf::kMaxSlots)
         return fail(RF_ERR_ARG, "slots=%d exceeds the %d lanes of a particle group", o.slots,
                     rf::kMaxSlots);
@@ -305,6 +352,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     // and so every list order, is unchanged.
     const int32_t gx = box->dims[0] + 1, gy = box->dims[1] + 1, gz = box->dims[2];
     const int64_t ncell = (int64_t)gx * gy * gz;
+    ALLOC(d.orig, n);
     ALLOC(d.ckey, n);
     ALLOC(d.ccount, ncell);
     ALLOC(d.cstart, ncell);
@@ -321,18 +369,30 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
 
     // particle state: (x, y, z, r), (0, 0, 0, m), (0, 0, 0, 0); per-particle
     // dt/m and dt/I as md_data_types.h:159-163, 185-186.
+    // stored particle p is the caller's particle orig[p]
+    if (o.order == RF_ORDER_SPATIAL) {
+        c->orig = spatial_order(s, box->cell);
+    } else {
+        c->orig.resize(n);
+        for (int64_t i = 0; i < n; ++i) c->orig[i] = (int32_t)i;
+    }
+    c->inv.resize(n);
+    for (int64_t q = 0; q < n; ++q) c->inv[c->orig[q]] = (int32_t)q;
     std::vector<double4> hp(n), hv(n), hw(n);
     std::vector<double2> hk(n);
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t i = c->orig[q];
         const double r = s->r[i];
         const double pV = 4.0 * (1.0 / 3.0) * kPi * r * r * r;
         const double pm = mat->density * pV;
         const double pI = 0.4 * pm * r * r;
-        hp[i] = make_double4(s->x[i], s->y[i], s->z[i], r);
-        hv[i] = make_double4(0.0, 0.0, 0.0, pm);
-        hw[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-        hk[i] = make_double2((1.0 / pm) * c->der.dt, c->der.dt / pI);
+        hp[q] = make_double4(s->x[i], s->y[i], s->z[i], r);
+        hv[q] = make_double4(0.0, 0.0, 0.0, pm);
+        hw[q] = make_double4(0.0, 0.0, 0.0, 0.0);
+        hk[q] = make_double2((1.0 / pm) * c->der.dt, c->der.dt / pI);
     }
+    if (hipMemcpy(d.orig, c->orig.data(), n * sizeof(int32_t), hipMemcpyHostToDevice))
+        return bail(fail(RF_ERR_HIP, "upload of particle ids failed"));
     for (int b = 0; b < 2; ++b) {
         if (hipMemcpy(d.pos[b], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice) ||
             hipMemcpy(d.vel[b], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice) ||
@@ -379,6 +439,7 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     p.dnsave = 1;
     p.store_fm = 1;
     p.curve_cap = 0;
+    p.zero_idx = c->inv[0];
     std::memset(&c->hctrl, 0, sizeof(c->hctrl));
     if (hipMemcpy(c->ctrl, &c->hctrl, sizeof(rf::Ctrl), hipMemcpyHostToDevice))
         return bail(fail(RF_ERR_HIP, "upload of control block failed"));
@@ -561,8 +622,8 @@ rf_status rf_build_bonds(rf_ctx* c) {
     std::vector<double> sc(n * B, 0.0), tc(n * B, 0.0);
     for (int64_t i = 0; i < n; ++i)
         for (int b = 0; b < nb[i]; ++b) {
-            const int64_t j = part[i * B + b];
-            const int64_t lo = std::min(i, j), hi = std::max(i, j);
+            const int64_t oi = c->orig[i], oj = c->orig[part[i * B + b]];
+            const int64_t lo = std::min(oi, oj), hi = std::max(oi, oj);
             sc[i * B + b] = bond_strength(c->mat.strength_seed, lo, hi, 0, c->mat.sigma_c,
                                           c->mat.sigma_c_cv);
             tc[i * B + b] = bond_strength(c->mat.strength_seed, lo, hi, 1, c->mat.tau_c,
@@ -689,6 +750,18 @@ rf_status rf_set_chunk(rf_ctx* c, int32_t steps) {
     c->chunk = steps;
     c->chunk_eff = steps;
     return RF_OK;
+}
+
+rf_status rf_capture_chunk(rf_ctx* c, int32_t steps) {
+    if (!c || steps <= 0) return fail(RF_ERR_ARG, "rf_capture_chunk: bad arguments");
+    if (!c->built) return fail(RF_ERR_STATE, "rf_capture_chunk before rf_build_bonds");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->chunk = steps;
+    c->chunk_eff = steps;
+    if (!c->use_graph || steps < 2) return RF_OK;
+    rf_ctx::Graph* g = nullptr;
+    return ensure_graph(c, c->timing ? steps - 1 : steps, &g);
 }
 
 rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
@@ -1012,12 +1085,14 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
     HIP_TRY(hipMemcpy(hw.data(), d.angv[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(hf.data(), d.F, n * sizeof(double4), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(hm.data(), d.M, n * sizeof(double4), hipMemcpyDeviceToHost));
-    auto split = [n](double* dst, const std::vector<double4>& src) {
+    const std::vector<int32_t>& og = c->orig;  // caller id of stored particle
+    auto split = [n, &og](double* dst, const std::vector<double4>& src) {
         if (!dst) return;
         for (int64_t i = 0; i < n; ++i) {
-            dst[i] = src[i].x;
-            dst[n + i] = src[i].y;
-            dst[2 * n + i] = src[i].z;
+            const int64_t o = og[i];
+            dst[o] = src[i].x;
+            dst[n + o] = src[i].y;
+            dst[2 * n + o] = src[i].z;
         }
     };
     split(v->R, hp); split(v->V, hv); split(v->W, hw); split(v->F, hf); split(v->M, hm);
@@ -1052,18 +1127,20 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
         }
     }
     for (int64_t i = 0; i < n; ++i) {
+        const int64_t o = og[i];
         for (int64_t k = 0; k < S; ++k) {
-            const int64_t q = i * S + k;
+            const int64_t qs = i * S + k;  // stored slot
+            const int64_t q = o * S + k;   // the caller's slot
             const bool oc = (occ[i] >> k) & 1u;
             const bool bd = oc && ((bnd[i] >> k) & 1u);
-            if (v->nbr) v->nbr[q] = oc ? nbr[q] : -1;
+            if (v->nbr) v->nbr[q] = oc ? og[nbr[qs]] : -1;
             if (v->type) v->type[q] = bd ? 1 : 0;
             if (v->bidx) v->bidx[q] = -1;
             const bool onz = oc && !bd && ((otnz[i] >> k) & 1u);
             if (v->o_t) {
-                v->o_t[3 * q] = onz ? ot[q].x : 0.0;
-                v->o_t[3 * q + 1] = onz ? ot[q].y : 0.0;
-                v->o_t[3 * q + 2] = onz ? ot[q].z : 0.0;
+                v->o_t[3 * q] = onz ? ot[qs].x : 0.0;
+                v->o_t[3 * q + 1] = onz ? ot[qs].y : 0.0;
+                v->o_t[3 * q + 2] = onz ? ot[qs].z : 0.0;
             }
             if (v->inv_L0) v->inv_L0[q] = 0.0;
             if (v->sigma_c) v->sigma_c[q] = 0.0;
@@ -1078,7 +1155,7 @@ rf_status rf_get_state(rf_ctx* c, rf_state_view* v) {
             const int64_t qb = i * B + b;
             const int ks = bslot[qb];
             if (ks < 0 || !((bnd[i] >> ks) & 1u) || !((occ[i] >> ks) & 1u)) continue;
-            const int64_t q = i * S + ks;
+            const int64_t q = o * S + ks;
             const rf::BondRec& r = brec[qb];
             if (v->bidx) v->bidx[q] = (int8_t)b;
             if (v->inv_L0) v->inv_L0[q] = r.invL0;
@@ -1110,13 +1187,15 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
     std::vector<double4> hp(n), hv(n), hw(n), hf(n), hm(n);
     HIP_TRY(hipMemcpy(hp.data(), d.pos[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(hv.data(), d.vel[cur], n * sizeof(double4), hipMemcpyDeviceToHost));
+    const std::vector<int32_t>& og = c->orig;  // caller id of stored particle
     for (int64_t i = 0; i < n; ++i) {
-        hp[i] = make_double4(v->R[i], v->R[n + i], v->R[2 * n + i], hp[i].w);
-        hv[i] = make_double4(v->V[i], v->V[n + i], v->V[2 * n + i], hv[i].w);
-        hw[i] = make_double4(v->W[i], v->W[n + i], v->W[2 * n + i], 0.0);
-        hf[i] = v->F ? make_double4(v->F[i], v->F[n + i], v->F[2 * n + i], 0.0)
+        const int64_t o = og[i];
+        hp[i] = make_double4(v->R[o], v->R[n + o], v->R[2 * n + o], hp[i].w);
+        hv[i] = make_double4(v->V[o], v->V[n + o], v->V[2 * n + o], hv[i].w);
+        hw[i] = make_double4(v->W[o], v->W[n + o], v->W[2 * n + o], 0.0);
+        hf[i] = v->F ? make_double4(v->F[o], v->F[n + o], v->F[2 * n + o], 0.0)
                      : make_double4(0.0, 0.0, 0.0, 0.0);
-        hm[i] = v->M ? make_double4(v->M[i], v->M[n + i], v->M[2 * n + i], 0.0)
+        hm[i] = v->M ? make_double4(v->M[o], v->M[n + o], v->M[2 * n + o], 0.0)
                      : make_double4(0.0, 0.0, 0.0, 0.0);
     }
     HIP_TRY(hipMemcpy(d.pos[cur], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
@@ -1132,18 +1211,19 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
     std::memset(brec.data(), 0, brec.size() * sizeof(rf::BondRec));
     for (int64_t i = 0; i < n; ++i)
         for (int64_t k = 0; k < S; ++k) {
-            const int64_t q = i * S + k;
+            const int64_t qs = i * S + k;            // stored slot
+            const int64_t q = (int64_t)og[i] * S + k;  // the caller's slot
             if (v->nbr[q] < 0) continue;
             if (v->nbr[q] >= n) return fail(RF_ERR_ARG, "slot partner out of range");
             occ[i] |= 1u << k;
-            nbr[q] = v->nbr[q];
+            nbr[qs] = c->inv[v->nbr[q]];
             if (v->type[q] == 1) {
                 const int b = v->bidx[q];
                 if (b < 0 || b >= B) return fail(RF_ERR_ARG, "bond slot without a valid bidx");
                 bnd[i] |= 1u << k;
                 const int64_t qb = i * B + b;
                 bslot[qb] = (int8_t)k;
-                bpart[qb] = v->nbr[q];
+                bpart[qb] = c->inv[v->nbr[q]];
                 nbond[i] = std::max(nbond[i], b + 1);
                 rf::BondRec& r = brec[qb];
                 r.invL0 = v->inv_L0 ? v->inv_L0[q] : 0.0;
@@ -1161,7 +1241,7 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
                     *mt[cc] = v->m_t ? v->m_t[3 * q + cc] : 0.0;
                 }
             } else if (v->o_t) {
-                ot[q] = make_double4(v->o_t[3 * q], v->o_t[3 * q + 1], v->o_t[3 * q + 2], 0.0);
+                ot[qs] = make_double4(v->o_t[3 * q], v->o_t[3 * q + 1], v->o_t[3 * q + 2], 0.0);
                 otnz[i] |= 1u << k;
             }
         }

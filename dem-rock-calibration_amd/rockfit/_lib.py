@@ -30,7 +30,7 @@ EXPORTS = [
     "rf_sens_material_tangent", "rf_sens_begin", "rf_sens_step", "rf_sens_get_curve",
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
     "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning", "rf_run_uniaxial",
-    "rf_run_brazilian", "rf_adjoint_run", "rf_batch_set_form",
+    "rf_run_brazilian", "rf_adjoint_run", "rf_batch_set_form", "rf_capture_chunk",
 ]
 
 
@@ -128,6 +128,8 @@ def lib():
     L.rf_fire_relax.restype = st
     L.rf_set_chunk.argtypes = [vp, C.c_int32]
     L.rf_set_chunk.restype = st
+    L.rf_capture_chunk.argtypes = [vp, C.c_int32]
+    L.rf_capture_chunk.restype = st
     _lib = L
     return L
 

@@ -25,6 +25,8 @@ STATUS_NAMES = {
 RF_QUIRK_BEND_DOUBLE_SCALE = 0x1
 RF_QUIRK_ROLL_RESET = 0x2
 RF_QUIRK_SLOT0_SENTINEL = 0x4
+RF_ORDER_SPATIAL = 0
+RF_ORDER_GIVEN = 1
 RF_QUIRKS_REFERENCE = 0x7
 
 RF_TEST_NONE = 0
@@ -60,7 +62,8 @@ class Sample(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("slots", C.c_int32), ("cand_cap", C.c_int32), ("skin", C.c_double),
-                ("device", C.c_int32), ("block", C.c_int32), ("stream", C.c_void_p)]
+                ("device", C.c_int32), ("block", C.c_int32), ("stream", C.c_void_p),
+                ("order", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Derived(C.Structure):

@@ -10,6 +10,7 @@ the uniaxial/Brazilian/batch/adjoint configurations (bond E, contact E,
 lognormal strengths, bond radius ratio).
 """
 import math
+import os
 
 import numpy as np
 
@@ -166,18 +167,24 @@ WORKLOADS = {
 
 def make_workload(name=None, n=None, r_min=0.2, r_max=0.3, shape="cylinder", aspect=2.0,
                   test="uniaxial", seed=20261203, gen_kwargs=None, mat_kwargs=None,
-                  test_kwargs=None, generator="fcc"):
+                  test_kwargs=None, generator="fcc", cache=None):
     """Generate a sample and build everything a test run needs.
 
     generator "rsi": random sequential insertion then FIRE on the GPU
     (sample.generate_rsi), the loading configurations' generator; "fcc":
-    the CPU jittered-lattice generator (sample.generate)."""
+    the CPU jittered-lattice generator (sample.generate).  cache: an .npz
+    path that holds the generator's output (written on first use, read
+    after), so runs of one workload in separate processes skip the
+    generation."""
     if name is not None:
         w = dict(WORKLOADS[name])
         n = n if n is not None else w["n"]
         r_min, r_max, shape, aspect, test = (w["r_min"], w["r_max"], w["shape"], w["aspect"],
                                              w["test"])
-    if generator == "rsi":
+    if cache is not None and os.path.exists(cache):
+        with np.load(cache) as z:
+            xyz, r = z["xyz"].copy(), z["r"].copy()
+    elif generator == "rsi":
         gk = dict(packing=0.6)
         gk.update(gen_kwargs or {})
         xyz, r, _ = sample.generate_rsi(n, r_min, r_max, shape=shape, aspect=aspect, seed=seed,
@@ -186,6 +193,8 @@ def make_workload(name=None, n=None, r_min=0.2, r_max=0.3, shape="cylinder", asp
         gk = dict(packing=0.6, tol=0.02, jitter=0.05, max_iter=300)
         gk.update(gen_kwargs or {})
         xyz, r = sample.generate(n, r_min, r_max, shape=shape, aspect=aspect, seed=seed, **gk)
+    if cache is not None and not os.path.exists(cache):
+        np.savez(cache, xyz=xyz, r=r)
     mat = material(r_min, 0.5 * (r_min + r_max), **(mat_kwargs or {}))
     factor = (1.6, 1.6, 1.2) if shape == "cylinder" else (1.3, 1.3, 1.5)
     xyz, box = place_in_box(xyz, r, mat, factor)
@@ -208,12 +217,13 @@ BRAZILIAN_SPEED = 0.02   # m/s, platens of the Brazilian configuration
 SEED = 20261203
 
 
-def config_workload(name, n=None, seed=None, dnsave=100, **test_kwargs):
+def config_workload(name, n=None, seed=None, dnsave=100, cache=None, **test_kwargs):
     """C1 (5k uniaxial), C2 (100k uniaxial) or C3 (50k Brazilian disc) with
     the configuration's loading: platens closing at strain rate 0.1/s times
     the specimen height (uniaxial) or at 0.02 m/s (Brazilian)."""
     idx = {"C1": 1, "C2": 2, "C3": 3}[name]
-    w = make_workload(name, n=n, seed=SEED + idx if seed is None else seed, generator="rsi")
+    w = make_workload(name, n=n, seed=SEED + idx if seed is None else seed, generator="rsi",
+                      cache=cache)
     if WORKLOADS[name]["test"] == "uniaxial":
         v = -STRAIN_RATE * w["cfg"].size[2] * 1e-3
         w["cfg"] = uniaxial_cfg(w["xyz"], w["r"], v_load_mps=v, dnsave=dnsave, **test_kwargs)
@@ -266,3 +276,18 @@ def gradient_workload(n=20000, steps=2000, seed=SEED + 5, dnsave=50):
     w["cfg"].v_load = -0.001 * w["cfg"].size[2] / (steps * m.dt)
     w["steps"] = steps
     return w
+
+
+def workload_hash(w):
+    """Hex digest of everything a loading run's result depends on: sphere
+    positions and radii, material, box and platen configuration."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for a in (np.ascontiguousarray(w["xyz"], dtype=np.float64),
+              np.ascontiguousarray(w["r"], dtype=np.float64)):
+        h.update(a.tobytes())
+    for s_ in (w["mat"], w["box"], w["cfg"]):
+        if s_ is not None:
+            h.update(bytes(s_))
+    return h.hexdigest()[:16]

@@ -81,9 +81,11 @@ class Context:
     """One sample on the GPU."""
 
     def __init__(self, xyz, r, mat, box, slots=32, cand_cap=32, skin=-1.0, device=0,
-                 stream=None):
+                 stream=None, order=abi.RF_ORDER_SPATIAL):
         """stream: a hipStream_t (as an int handle, e.g. torch.cuda.Stream's
-        cuda_stream) the context runs on instead of its own."""
+        cuda_stream) the context runs on instead of its own.  order: how the
+        context stores the particles (RF_ORDER_SPATIAL or RF_ORDER_GIVEN);
+        every call speaks in the caller's ids either way."""
         self.n = int(len(r))
         self.slots = int(slots)
         self._x = np.ascontiguousarray(xyz[:, 0], dtype=np.float64)
@@ -93,7 +95,7 @@ class Context:
         dp = C.POINTER(C.c_double)
         s = abi.Sample(self.n, self._x.ctypes.data_as(dp), self._y.ctypes.data_as(dp),
                        self._z.ctypes.data_as(dp), self._r.ctypes.data_as(dp))
-        opt = abi.Options(slots, cand_cap, skin, device, 256, stream)
+        opt = abi.Options(slots, cand_cap, skin, device, 256, stream, order, 0)
         h = C.c_void_p()
         self.h = None
         check(lib().rf_create(C.byref(s), C.byref(mat), C.byref(box), C.byref(opt), C.byref(h)),
@@ -225,6 +227,11 @@ class Context:
 
     def set_chunk(self, steps):
         check(lib().rf_set_chunk(self.h, int(steps)), "rf_set_chunk")
+
+    def capture_chunk(self, steps):
+        """Set the chunk length and capture its graph now (rf_capture_chunk);
+        no step runs."""
+        check(lib().rf_capture_chunk(self.h, int(steps)), "rf_capture_chunk")
 
     def last_timing(self):
         """(summed ms of the bracketed step kernels, bracketed launches,

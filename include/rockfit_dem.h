@@ -97,6 +97,19 @@ typedef struct rf_sample {
     const double* r;
 } rf_sample;
 
+/* Order in which a context stores its particles (rf_options.order).
+ * SPATIAL: sorted by the Morton code of (floor(4 x / cell), floor(4 y / cell),
+ * floor(4 z / cell)) at creation, ties by id, so that particles close in
+ * space sit close in memory and the partner gathers of a block of particles
+ * hit the same cache lines; GIVEN: the caller's order.  The interface always
+ * speaks in the caller's ids.  The order changes nothing but the grouping of
+ * the platen-force sum (blocks of 32 particles in storage order, the only
+ * per-particle sum across particles), so results of the two orders agree to
+ * rounding and each is reproduced bit for bit by the CPU restatement run in
+ * the same order. */
+#define RF_ORDER_SPATIAL 0
+#define RF_ORDER_GIVEN 1
+
 typedef struct rf_options {
     int32_t slots;        /* interaction slots per particle (IonP), default 32  */
     int32_t cand_cap;     /* neighbour-candidate capacity per particle, <= 32 (GPU) */
@@ -108,6 +121,8 @@ typedef struct rf_options {
                              created on `device` (not the null stream), kept
                              by the caller and not destroyed by rf_destroy;
                              NULL: the context creates and owns its stream */
+    int32_t order;        /* RF_ORDER_SPATIAL (default) or RF_ORDER_GIVEN        */
+    int32_t pad_;
 } rf_options;
 
 /* Derived parameters (potential_data::CalculateParameters, md_data_types.h:155-187). */
@@ -297,6 +312,14 @@ rf_status rf_last_timing(rf_ctx* ctx, double* step_kernel_ms, int64_t* timed_lau
 
 /* Steps per host synchronisation inside rf_step (default 256). */
 rf_status rf_set_chunk(rf_ctx* ctx, int32_t steps_per_sync);
+
+/* Set the chunk length (as rf_set_chunk) and capture now the HIP graph that
+ * rf_step replays for a full chunk of that length (with timing on, the
+ * chunk's first steps - 1 steps), so that the first rf_step does not pay for
+ * the capture.  Runs no step.  Call after rf_set_test and rf_set_timing (the
+ * graph bakes in the test's constants; they are re-captured if these change).
+ * No counterpart in the reference. */
+rf_status rf_capture_chunk(rf_ctx* ctx, int32_t steps);
 
 /* Replay each full chunk of rf_step as one captured HIP graph (default on).
  * With timing on, the last step of each graphed chunk is launched on its own

@@ -72,6 +72,7 @@ struct ora_ctx {
     int64_t* bcount;                    /* [cap]                               */
     int64_t curve_cap;
     double *part_b, *part_t;            /* per-particle platen contributions   */
+    int64_t* ord;                       /* particle at place p of the platen-sum order */
     int64_t nblk;
     double *blk_b, *blk_t;              /* block partials                      */
     char err[256];
@@ -190,6 +191,69 @@ double ora_bond_strength(uint64_t seed, int64_t lo, int64_t hi, int32_t stream, 
 /* ---------------------------------------------------------- create ---- */
 static void* zalloc(size_t n) { return calloc(n ? n : 1, 1); }
 
+/* Order of the platen-force sum.  The device stores particles sorted by the
+ * Morton code of their quarter-cell coordinates q = floor(4 x / cell) (21 bits
+ * each; bit 3b of the code is bit b of qx, 3b+1 of qy, 3b+2 of qz), ties by
+ * id (RF_ORDER_SPATIAL), or in id order (RF_ORDER_GIVEN), and sums the
+ * per-particle platen forces over blocks of 32 stored particles.  Summing
+ * over the same blocks is the only place where the storage order reaches a
+ * result.  Pinned by a numpy restatement (tests/test_oracle_lists.py). */
+static uint64_t spread_bits3(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+static uint64_t quarter(double x, double s) {
+    double q = floor(x * s);
+    if (!(q > 0.0)) return 0;
+    return q >= 2097151.0 ? 2097151u : (uint64_t)q;
+}
+
+typedef struct { uint64_t key; int64_t id; } order_key;
+
+static int cmp_order(const void* a, const void* b) {
+    const order_key* x = (const order_key*)a;
+    const order_key* y = (const order_key*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id);
+}
+
+static void set_order(ora_ctx* c, int32_t mode) {
+    const int64_t n = c->n;
+    if (mode == RF_ORDER_GIVEN) {
+        for (int64_t i = 0; i < n; ++i) c->ord[i] = i;
+        return;
+    }
+    const double s = 4.0 * (1.0 / c->box.cell);
+    order_key* k = (order_key*)malloc(n * sizeof(order_key));
+    for (int64_t i = 0; i < n; ++i) {
+        k[i].key = spread_bits3(quarter(c->R[i], s)) | (spread_bits3(quarter(c->R[n + i], s)) << 1) |
+                   (spread_bits3(quarter(c->R[2 * n + i], s)) << 2);
+        k[i].id = i;
+    }
+    qsort(k, n, sizeof(order_key), cmp_order);
+    for (int64_t p = 0; p < n; ++p) c->ord[p] = k[p].id;
+    free(k);
+}
+
+/* Storage order the device uses for this sample (default RF_ORDER_SPATIAL);
+ * call before the first step, while R holds the sample's positions. */
+int ora_set_order(ora_ctx* c, int32_t mode) {
+    if (mode != RF_ORDER_SPATIAL && mode != RF_ORDER_GIVEN) return RF_ERR_ARG;
+    set_order(c, mode);
+    return RF_OK;
+}
+
+int ora_get_order(const ora_ctx* c, int64_t* out) {
+    memcpy(out, c->ord, c->n * sizeof(int64_t));
+    return RF_OK;
+}
+
 ora_ctx* ora_create(const rf_sample* s, const rf_material* mat, const rf_box* box, int32_t slots,
                     int32_t nthreads) {
     if (!s || !mat || !box || s->n <= 0 || slots <= 0) return NULL;
@@ -224,6 +288,7 @@ ora_ctx* ora_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     c->ncell = (int64_t)c->box.dims[0] * c->box.dims[1] * c->box.dims[2];
     c->cstart = zalloc(c->ncell * sizeof(int32_t)); c->ccount = zalloc(c->ncell * sizeof(int32_t));
     c->part_b = zalloc(n * sizeof(double)); c->part_t = zalloc(n * sizeof(double));
+    c->ord = zalloc(n * sizeof(int64_t));
     c->nblk = (n + RED_BLOCK - 1) / RED_BLOCK;
     c->blk_b = zalloc(c->nblk * sizeof(double)); c->blk_t = zalloc(c->nblk * sizeof(double));
     double rmin = s->r[0], rmax = s->r[0];
@@ -248,6 +313,7 @@ ora_ctx* ora_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     c->acut2 = mat->acut * mat->acut;
     c->inv_a = 1.0 / box->cell;
     c->has_test = 0;
+    set_order(c, RF_ORDER_SPATIAL);
     return c;
 }
 
@@ -257,7 +323,7 @@ void ora_destroy(ora_ctx* c) {
     free(c->rad); free(c->mass); free(c->dtdm); free(c->dtdI);
     free(c->nbr); free(c->type); free(c->bidx); free(c->invL0); free(c->sc); free(c->tc);
     free(c->pn); free(c->ot); free(c->mn); free(c->mt);
-    free(c->key); free(c->sorted); free(c->cstart); free(c->ccount);
+    free(c->key); free(c->sorted); free(c->cstart); free(c->ccount); free(c->ord);
     free(c->part_b); free(c->part_t); free(c->blk_b); free(c->blk_t);
     free(c->curve); free(c->bcount);
     free(c);
@@ -929,12 +995,12 @@ static void reduce_partials(ora_ctx* c) {
     for (int64_t b = 0; b < c->nblk; ++b) {
         for (int t = 0; t < RED_BLOCK; ++t) {
             int64_t i = b * RED_BLOCK + t;
-            buf[t] = i < c->n ? c->part_b[i] : -0.0;
+            buf[t] = i < c->n ? c->part_b[c->ord[i]] : -0.0;
         }
         c->blk_b[b] = tree_n(buf, RED_BLOCK);
         for (int t = 0; t < RED_BLOCK; ++t) {
             int64_t i = b * RED_BLOCK + t;
-            buf[t] = i < c->n ? c->part_t[i] : -0.0;
+            buf[t] = i < c->n ? c->part_t[c->ord[i]] : -0.0;
         }
         c->blk_t[b] = tree_n(buf, RED_BLOCK);
     }

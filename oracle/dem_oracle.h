@@ -29,6 +29,11 @@ void ora_destroy(ora_ctx* c);
 const char* ora_error(const ora_ctx* c);
 int64_t ora_sizeof(int32_t which);
 void ora_set_threads(ora_ctx* c, int32_t nthreads);
+/* Storage order of the device (RF_ORDER_SPATIAL, the default, or
+ * RF_ORDER_GIVEN): the grouping of the platen-force sum.  Call before the
+ * first step. */
+int ora_set_order(ora_ctx* c, int32_t mode);
+int ora_get_order(const ora_ctx* c, int64_t* out);
 
 /* Cell keys, stable (cell, id) order, cell start/count (CellDistribution). */
 int ora_cells(ora_ctx* c);

@@ -53,6 +53,8 @@ def lib():
     L.ora_error.argtypes = [vp]
     L.ora_error.restype = C.c_char_p
     L.ora_set_threads.argtypes = [vp, C.c_int32]
+    L.ora_set_order.argtypes = [vp, C.c_int32]
+    L.ora_get_order.argtypes = [vp, vp]
     for name in ("ora_cells", "ora_build_bonds", "ora_construct_rows", "ora_construct_bonds",
                  "ora_assign_strengths", "ora_append", "ora_prune", "ora_forces",
                  "ora_integrate", "ora_post"):
@@ -141,7 +143,7 @@ def fire_relax(xyz, r, cfg, vel=None, threads=1):
 class Oracle:
     """One sample in the CPU oracle; mirrors the product's Context API."""
 
-    def __init__(self, xyz, r, mat, box, slots=32, threads=None):
+    def __init__(self, xyz, r, mat, box, slots=32, threads=None, order=0):
         abi = _abi()
         self.abi = abi
         self.n = len(r)
@@ -159,6 +161,8 @@ class Oracle:
         self.h = lib().ora_create(C.byref(s), C.byref(mat), C.byref(box), slots, th)
         if not self.h:
             raise RuntimeError("ora_create failed")
+        if order:
+            self._check(lib().ora_set_order(self.h, order), "set_order")
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -169,6 +173,12 @@ class Oracle:
     def _check(self, rc, what):
         if rc:
             raise RuntimeError(f"oracle {what}: status {rc}: {lib().ora_error(self.h).decode()}")
+
+    def order(self):
+        """Particles in the order the platen-force sum groups them."""
+        out = np.zeros(self.n, dtype=np.int64)
+        lib().ora_get_order(self.h, _ptr(out))
+        return out
 
     def set_threads(self, t):
         lib().ora_set_threads(self.h, t)

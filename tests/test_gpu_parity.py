@@ -16,9 +16,9 @@ from rockfit.core import Context
 pytestmark = pytest.mark.gpu
 
 
-def _pair(w, slots=32, skin=-1.0):
-    g = Context(w["xyz"], w["r"], w["mat"], w["box"], slots=slots, skin=skin)
-    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], slots=slots, threads=4)
+def _pair(w, slots=32, skin=-1.0, order=abi.RF_ORDER_SPATIAL):
+    g = Context(w["xyz"], w["r"], w["mat"], w["box"], slots=slots, skin=skin, order=order)
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], slots=slots, threads=4, order=order)
     return g, o
 
 
@@ -103,22 +103,9 @@ def test_free_motion_append_parity():
     cfg.kind = abi.RF_TEST_NONE
     g.set_test(cfg)
     o.set_test(cfg)
-    prev = o.get_state()
     for chunk in (1, 20, 80):
         assert g.step(chunk) == o.step(chunk) == chunk
-        gs, os_ = g.get_state(), o.get_state()
-        try:
-            compare_states(gs, os_, g.n, g.slots, what=f"free motion +{chunk}")
-        except AssertionError:
-            import os
-            out = os.path.join(os.environ.get("GRAFT_REPO_ROOT", "."), "gpurun_out")
-            if os.path.isdir(out):
-                np.savez_compressed(os.path.join(out, "free_motion_diverge.npz"),
-                                    g_nbr=gs["nbr"], o_nbr=os_["nbr"], R=os_["R"], V=os_["V"],
-                                    W=os_["W"], prev_R=prev["R"], prev_nbr=prev["nbr"],
-                                    xyz=w["xyz"], r=w["r"])
-            raise
-        prev = os_
+        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"free motion +{chunk}")
     c = g.counters()
     assert c.rebuilds > 2
 
@@ -358,4 +345,27 @@ def test_high_slots_parity():
     assert g.step(400) == o.step(400) == 400
     compare_states(g.get_state(), o.get_state(), g.n, g.slots, what="high slots, 400 steps")
     compare_test_states(g.test_state(), o.test_state(), what="high slots, 400 steps")
+    np.testing.assert_array_equal(g.curve(), o.curve())
+
+
+def test_given_order_parity():
+    """The caller's particle order kept as the storage order (RF_ORDER_GIVEN)
+    instead of the spatial one: random motion with appends and prunes, and a
+    platen test, equal to the oracle run in the same order."""
+    w = config.make_workload(n=1500, r_min=0.2, r_max=0.3, test="uniaxial", seed=13,
+                             mat_kwargs=dict(sigma_mpa=5.0, tau_mpa=8.0),
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=5, max_strain=0.05))
+    print("case given order n=1500", flush=True)
+    g, o = _pair(w, skin=0.02, order=abi.RF_ORDER_GIVEN)
+    g.build_bonds()
+    o.build_bonds()
+    st = randomize_motion(o.get_state(), g.n, 11, 1e-3, 1e-2)
+    g.set_state(st)
+    o.set_state(st)
+    g.set_test(w["cfg"])
+    o.set_test(w["cfg"])
+    for chunk in (1, 30, 120):
+        assert g.step(chunk) == o.step(chunk) == chunk
+        compare_states(g.get_state(), o.get_state(), g.n, g.slots, what=f"given order +{chunk}")
+        compare_test_states(g.test_state(), o.test_state(), what=f"given order +{chunk}")
     np.testing.assert_array_equal(g.curve(), o.curve())

@@ -156,3 +156,34 @@ def test_row_overflow_is_an_error():
     o = pyoracle.Oracle(xyz, r, mat, box, slots=4)
     with pytest.raises(RuntimeError, match="overflow"):
         o.build_bonds()
+
+
+def _morton_order(xyz, cell):
+    s = 4.0 * (1.0 / cell)
+    q = np.clip(np.floor(xyz * s), 0, 2097151).astype(np.uint64)
+    key = np.zeros(len(xyz), dtype=np.uint64)
+    for b in range(21):
+        for a in range(3):
+            key |= ((q[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + a)
+    return np.lexsort((np.arange(len(xyz)), key))
+
+
+def test_platen_sum_order():
+    """The oracle groups the platen-force sum like the device stores its
+    particles: by the Morton code of the quarter-cell coordinates
+    floor(4 x / cell), ties by id (RF_ORDER_SPATIAL), or by id
+    (RF_ORDER_GIVEN).  Restated here with numpy bit operations."""
+    w = config.make_workload(n=1200, r_min=0.2, r_max=0.3, test="uniaxial", seed=4)
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"])
+    np.testing.assert_array_equal(o.order(), _morton_order(w["xyz"], w["box"].cell))
+    assert not np.array_equal(o.order(), np.arange(o.n))
+    g = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], order=abi.RF_ORDER_GIVEN)
+    np.testing.assert_array_equal(g.order(), np.arange(g.n))
+    # the two orders group the same platen forces differently: equal to rounding
+    cfg = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=-2.0, dnsave=1)
+    for x in (o, g):
+        x.build_bonds()
+        x.set_test(cfg)
+        x.step(150)
+    a, b = o.test_state(), g.test_state()
+    assert a.sfl1 != 0.0 and abs(a.sfl1 - b.sfl1) <= 1e-12 * abs(a.sfl1)
