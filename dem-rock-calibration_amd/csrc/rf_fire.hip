@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/rockfit_dem.h"
@@ -421,6 +422,34 @@ __global__ void k_fire_clear(FireCtrl* ctrl) {
     if (threadIdx.x == 0) ctrl->rebuild = 0;
 }
 
+// Sample in, sample out for the static-memory entry point: SoA host copies
+// staged in device memory, packed into / unpacked from the particle state.
+// The mass is the same expression, in the same order, as rf_fire_relax's
+// host-side one.
+__global__ void k_fire_pack(FireDev d, const double* x, const double* y, const double* z,
+                            const double* r, const double* v, int has_v, double density,
+                            int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double ri = r[i];
+    const double m = density * (4.0 / 3.0) * kPiF * ri * ri * ri;
+    d.pos[i] = make_double4(x[i], y[i], z[i], ri);
+    d.vel[i] = has_v ? make_double4(v[i], v[n + i], v[2 * n + i], m) : make_double4(0.0, 0.0, 0.0, m);
+    d.alive[i] = 1;
+}
+
+__global__ void k_fire_unpack(FireDev d, double* x, double* y, double* z, double* v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 q = d.pos[i], u = d.vel[i];
+    x[i] = q.x;
+    y[i] = q.y;
+    z[i] = q.z;
+    v[i] = u.x;
+    v[n + i] = u.y;
+    v[2 * n + i] = u.z;
+}
+
 unsigned grid_of(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 #define FIRE_TRY(expr)                                                                     \
@@ -457,33 +486,11 @@ rf_status build_lists(FireDev& d, const FireParams& p, FireCtrl* ctrl, hipStream
     return RF_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-rf_status rf_fire_defaults(rf_fire_cfg* c) {
-    if (!c) return rf::report(RF_ERR_ARG, "null cfg");
-    *c = rf_fire_cfg{};
-    c->alpha0 = 0.1;          // md_createcylindersample.cu:349
-    c->dtgrow = 1.1;          // SetFIREData, md_firerelaxation.cu:72-86
-    c->dtshrink = 0.5;
-    c->alpha_shrink = 0.99;
-    c->dtmax_factor = 100.0;
-    c->dtmin_factor = 0.001;
-    c->ndelay = 20;
-    c->np_negative_max = 30000;
-    c->gap_tol = -1e300;
-    c->check_every = 1;
-    c->delete_after = -1;
-    c->quirks = RF_QUIRK_FIRE_NO_INERTIA;
-    c->skin = -1.0;
-    return RF_OK;
-}
-
-rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
-                        const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out) {
-    if (out) *out = rf_fire_result{};
-    if (n <= 0 || !x || !y || !z || !r || !cfg || !out)
+// Checks the arguments and fills the run's constants from cfg and the
+// sample (host arrays).
+rf_status fire_setup(int64_t n, const double* x, const double* y, const double* z, const double* r,
+                     const rf_fire_cfg* cfg, FireParams& p) {
+    if (n <= 0 || !x || !y || !z || !r || !cfg)
         return rf::report(RF_ERR_ARG, "rf_fire_relax: bad arguments");
     if (n >= (int64_t)1 << 31) return rf::report(RF_ERR_ARG, "rf_fire_relax: n too large");
     if (cfg->law != RF_FIRE_LAW_REFERENCE && cfg->law != RF_FIRE_LAW_HARMONIC)
@@ -519,7 +526,7 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
     const double skin = cfg->skin >= 0.0 ? cfg->skin : 0.25 * rmax;
     const double cut = 2.0 * rmax + skin;
 
-    FireParams p{};
+    p = FireParams{};
     p.n = n;
     p.nblk = (int32_t)grid_of(n, kFB);
     p.K = kFList;
@@ -562,6 +569,98 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
         p.nc[k] = (int32_t)std::floor((hi[k] + 2.0 * cell - p.lo[k]) / cell) + 1;
         p.ncell *= p.nc[k];
     }
+    return RF_OK;
+}
+
+// The FIRE controller's starting state.
+FireCtrl fire_ctrl0(const rf_fire_cfg* cfg, const FireParams& p) {
+    FireCtrl hc{};
+    hc.dt = cfg->dt0;
+    hc.alpha = cfg->alpha0;
+    hc.min_gap = 0.0;
+    hc.del_j = -1;
+    if (!(0 < p.max_steps || hc.min_gap < p.gap_tol)) {
+        hc.halt = 1;
+        hc.reason = 1;
+    } else if (p.cap <= 0) {
+        hc.halt = 1;
+        hc.reason = 3;
+    }
+    return hc;
+}
+
+// The relaxation loop on stream `st`: 256 steps per host wait, lists
+// rebuilt when the device asks.  `hc` holds the uploaded starting state and
+// receives the final one.
+rf_status fire_loop(FireDev& d, const FireParams& p, FireCtrl* ctrl, FireCtrl& hc, hipStream_t st) {
+    const unsigned gb = (unsigned)p.nblk;
+    rf_status rs = build_lists(d, p, ctrl, st);
+    if (rs) return rs;
+    int64_t last_step = -1;
+    while (!hc.halt) {
+        for (int s = 0; s < 256; ++s) {
+            hipLaunchKernelGGL(k_fire_force, dim3(gb), dim3(kFB), 0, st, d, p, ctrl);
+            hipLaunchKernelGGL(k_fire_ctrl, dim3(1), dim3(kFC), 0, st, d, p, ctrl);
+            hipLaunchKernelGGL(k_fire_move, dim3(gb), dim3(kFB), 0, st, d, p, ctrl);
+        }
+        FIRE_TRY(hipGetLastError());
+        FIRE_TRY(hipMemcpyAsync(&hc, ctrl, sizeof hc, hipMemcpyDeviceToHost, st));
+        FIRE_TRY(hipStreamSynchronize(st));
+        if (hc.err) return rf::report(RF_ERR_OVERFLOW, "FIRE neighbour list longer than %d", kFList);
+        if (hc.halt) break;
+        if (hc.rebuild) {
+            rs = build_lists(d, p, ctrl, st);
+            if (rs) return rs;
+        } else if (hc.step == last_step) {
+            return rf::report(RF_ERR_STATE, "FIRE loop halted without a reason");
+        }
+        last_step = hc.step;
+    }
+    return RF_OK;
+}
+
+void fire_result(const FireCtrl& hc, rf_fire_result* out) {
+    out->steps = hc.step;
+    out->np_positive = hc.npos;
+    out->np_negative = hc.nneg;
+    out->removed = hc.removed;
+    out->dt = hc.dt;
+    out->alpha = hc.alpha;
+    out->min_gap = hc.min_gap;
+    out->power = hc.power;
+    out->stop_reason = hc.reason;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_fire_defaults(rf_fire_cfg* c) {
+    if (!c) return rf::report(RF_ERR_ARG, "null cfg");
+    *c = rf_fire_cfg{};
+    c->alpha0 = 0.1;          // md_createcylindersample.cu:349
+    c->dtgrow = 1.1;          // SetFIREData, md_firerelaxation.cu:72-86
+    c->dtshrink = 0.5;
+    c->alpha_shrink = 0.99;
+    c->dtmax_factor = 100.0;
+    c->dtmin_factor = 0.001;
+    c->ndelay = 20;
+    c->np_negative_max = 30000;
+    c->gap_tol = -1e300;
+    c->check_every = 1;
+    c->delete_after = -1;
+    c->quirks = RF_QUIRK_FIRE_NO_INERTIA;
+    c->skin = -1.0;
+    return RF_OK;
+}
+
+rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
+                        const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out) {
+    if (out) *out = rf_fire_result{};
+    if (!out) return rf::report(RF_ERR_ARG, "rf_fire_relax: bad arguments");
+    FireParams p;
+    rf_status rs = fire_setup(n, x, y, z, r, cfg, p);
+    if (rs) return rs;
 
     FireMem mem;
     FireDev d{};
@@ -597,18 +696,7 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
         FIRE_TRY(hipMemcpy(d.vel, hv.data(), n * sizeof(double4), hipMemcpyHostToDevice));
         FIRE_TRY(hipMemcpy(d.alive, ha.data(), n, hipMemcpyHostToDevice));
     }
-    FireCtrl hc{};
-    hc.dt = cfg->dt0;
-    hc.alpha = cfg->alpha0;
-    hc.min_gap = 0.0;
-    hc.del_j = -1;
-    if (!(0 < p.max_steps || hc.min_gap < p.gap_tol)) {
-        hc.halt = 1;
-        hc.reason = 1;
-    } else if (p.cap <= 0) {
-        hc.halt = 1;
-        hc.reason = 3;
-    }
+    FireCtrl hc = fire_ctrl0(cfg, p);
     FIRE_TRY(hipMemcpy(ctrl, &hc, sizeof hc, hipMemcpyHostToDevice));
 
     hipStream_t st = nullptr;
@@ -619,29 +707,8 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
             if (s) hipStreamDestroy(s);
         }
     } sg{st};
-    const unsigned gb = (unsigned)p.nblk;
-    rf_status rs = build_lists(d, p, ctrl, st);
+    rs = fire_loop(d, p, ctrl, hc, st);
     if (rs) return rs;
-    int64_t last_step = -1;
-    while (!hc.halt) {
-        for (int s = 0; s < 256; ++s) {
-            hipLaunchKernelGGL(k_fire_force, dim3(gb), dim3(kFB), 0, st, d, p, ctrl);
-            hipLaunchKernelGGL(k_fire_ctrl, dim3(1), dim3(kFC), 0, st, d, p, ctrl);
-            hipLaunchKernelGGL(k_fire_move, dim3(gb), dim3(kFB), 0, st, d, p, ctrl);
-        }
-        FIRE_TRY(hipGetLastError());
-        FIRE_TRY(hipMemcpyAsync(&hc, ctrl, sizeof hc, hipMemcpyDeviceToHost, st));
-        FIRE_TRY(hipStreamSynchronize(st));
-        if (hc.err) return rf::report(RF_ERR_OVERFLOW, "FIRE neighbour list longer than %d", kFList);
-        if (hc.halt) break;
-        if (hc.rebuild) {
-            rs = build_lists(d, p, ctrl, st);
-            if (rs) return rs;
-        } else if (hc.step == last_step) {
-            return rf::report(RF_ERR_STATE, "FIRE loop halted without a reason");
-        }
-        last_step = hc.step;
-    }
     {
         std::vector<double4> hp(n), hv(n);
         FIRE_TRY(hipMemcpy(hp.data(), d.pos, n * sizeof(double4), hipMemcpyDeviceToHost));
@@ -657,15 +724,128 @@ rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double
             }
         }
     }
-    out->steps = hc.step;
-    out->np_positive = hc.npos;
-    out->np_negative = hc.nneg;
-    out->removed = hc.removed;
-    out->dt = hc.dt;
-    out->alpha = hc.alpha;
-    out->min_gap = hc.min_gap;
-    out->power = hc.power;
-    out->stop_reason = hc.reason;
+    fire_result(hc, out);
+    return RF_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- static working memory --
+// rf_fire_relax_noalloc's working memory: one array per GPU, loaded with the
+// library's code object and held until the process ends.  Host code never
+// uses the name `scratch` as an address: each device's address comes from
+// hipGetSymbolAddress on that device's first call and is kept per device.
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];  // one call at a time per GPU
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's `scratch` (the device must be current;
+// the caller holds g_scratch_mu[dev]).
+rf_status scratch_base(int dev, unsigned char** out) {
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return rf::report(RF_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                              hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return RF_OK;
+}
+
+// Consecutive 256-byte-aligned pieces of one region; with base == nullptr
+// it only measures.
+struct Carve {
+    unsigned char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* q = base ? (T*)(base + off) : nullptr;
+        off += std::max<size_t>(count, 1) * sizeof(T);
+        return q;
+    }
+};
+
+// The run's arrays (and the SoA staging of the host copies) in one region.
+void fire_carve(Carve& m, const FireParams& p, FireDev& d, FireCtrl** ctrl, double** stage) {
+    const size_t n = (size_t)p.n;
+    d.pos = m.take<double4>(n);
+    d.vel = m.take<double4>(n);
+    d.frc = m.take<double4>(n);
+    d.rb = m.take<double4>(n);
+    d.alive = m.take<uint8_t>(n);
+    d.nb = m.take<int32_t>(n * kFList);
+    d.nn = m.take<int32_t>(n);
+    d.key = m.take<int32_t>(n);
+    d.cnt = m.take<int32_t>((size_t)p.ncell);
+    d.start = m.take<int32_t>((size_t)p.ncell);
+    d.fill = m.take<int32_t>((size_t)p.ncell);
+    d.sorted = m.take<int32_t>(n);
+    d.pw = m.take<double>((size_t)p.nblk);
+    d.gmin = m.take<double>((size_t)p.nblk);
+    d.gi = m.take<int32_t>((size_t)p.nblk);
+    d.gj = m.take<int32_t>((size_t)p.nblk);
+    *ctrl = m.take<FireCtrl>(1);
+    *stage = m.take<double>(7 * n);  // x, y, z, r, velocity (3 n)
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_fire_relax_noalloc(int64_t n, double* x, double* y, double* z, const double* r,
+                                const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out) {
+    if (out) *out = rf_fire_result{};
+    if (!out) return rf::report(RF_ERR_ARG, "rf_fire_relax_noalloc: bad arguments");
+    FireParams p;
+    rf_status rs = fire_setup(n, x, y, z, r, cfg, p);  // also makes cfg->device current
+    if (rs) return rs;
+    if (cfg->device >= kMaxDevices) return rf::report(RF_ERR_ARG, "device %d too high", cfg->device);
+    FireDev d{};
+    FireCtrl* ctrl = nullptr;
+    double* stage = nullptr;
+    Carve probe{nullptr};
+    fire_carve(probe, p, d, &ctrl, &stage);
+    if (probe.off > kScratchBytes)
+        return rf::report(RF_ERR_ARG,
+                          "rf_fire_relax_noalloc: %lld particles (%lld grid cells) need %zu bytes "
+                          "of working memory; the static array holds %zu",
+                          (long long)n, (long long)p.ncell, probe.off, kScratchBytes);
+    std::lock_guard<std::mutex> lock(g_scratch_mu[cfg->device]);
+    unsigned char* base = nullptr;
+    rs = scratch_base(cfg->device, &base);
+    if (rs) return rs;
+    Carve m{base};
+    fire_carve(m, p, d, &ctrl, &stage);
+    double *sx = stage, *sy = stage + n, *sz = stage + 2 * n, *sr = stage + 3 * n,
+           *sv = stage + 4 * n;
+    const hipStream_t st = nullptr;  // the null stream: no stream to create
+    FIRE_TRY(hipMemcpy(sx, x, n * sizeof(double), hipMemcpyHostToDevice));
+    FIRE_TRY(hipMemcpy(sy, y, n * sizeof(double), hipMemcpyHostToDevice));
+    FIRE_TRY(hipMemcpy(sz, z, n * sizeof(double), hipMemcpyHostToDevice));
+    FIRE_TRY(hipMemcpy(sr, r, n * sizeof(double), hipMemcpyHostToDevice));
+    if (velocity) FIRE_TRY(hipMemcpy(sv, velocity, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fire_pack, dim3(grid_of(n, 256)), dim3(256), 0, st, d, sx, sy, sz, sr, sv,
+                       velocity ? 1 : 0, cfg->density, n);
+    FIRE_TRY(hipGetLastError());
+    FireCtrl hc = fire_ctrl0(cfg, p);
+    FIRE_TRY(hipMemcpy(ctrl, &hc, sizeof hc, hipMemcpyHostToDevice));
+    rs = fire_loop(d, p, ctrl, hc, st);
+    if (rs) return rs;
+    hipLaunchKernelGGL(k_fire_unpack, dim3(grid_of(n, 256)), dim3(256), 0, st, d, sx, sy, sz, sv, n);
+    FIRE_TRY(hipGetLastError());
+    FIRE_TRY(hipMemcpy(x, sx, n * sizeof(double), hipMemcpyDeviceToHost));
+    FIRE_TRY(hipMemcpy(y, sy, n * sizeof(double), hipMemcpyDeviceToHost));
+    FIRE_TRY(hipMemcpy(z, sz, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (velocity) FIRE_TRY(hipMemcpy(velocity, sv, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    fire_result(hc, out);
     return RF_OK;
 }
 

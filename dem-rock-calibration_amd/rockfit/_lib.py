@@ -31,6 +31,7 @@ EXPORTS = [
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
     "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning", "rf_run_uniaxial",
     "rf_run_brazilian", "rf_adjoint_run", "rf_batch_set_form", "rf_capture_chunk",
+    "rf_fire_relax_noalloc",
 ]
 
 
@@ -126,6 +127,8 @@ def lib():
     L.rf_fire_relax.argtypes = [C.c_int64, P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_double), P(abi.FireCfg), P(C.c_double), P(abi.FireResult)]
     L.rf_fire_relax.restype = st
+    L.rf_fire_relax_noalloc.argtypes = L.rf_fire_relax.argtypes
+    L.rf_fire_relax_noalloc.restype = st
     L.rf_set_chunk.argtypes = [vp, C.c_int32]
     L.rf_set_chunk.restype = st
     L.rf_capture_chunk.argtypes = [vp, C.c_int32]

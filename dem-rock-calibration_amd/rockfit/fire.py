@@ -27,8 +27,10 @@ def defaults(**kw):
     return c
 
 
-def relax(xyz, r, cfg, vel=None):
-    """Relax positions xyz (n, 3) mm with radii r; returns (xyz, vel, result)."""
+def relax(xyz, r, cfg, vel=None, noalloc=False):
+    """Relax positions xyz (n, 3) mm with radii r; returns (xyz, vel, result).
+    noalloc: use rf_fire_relax_noalloc (working memory in the library's
+    static device array, no allocation)."""
     xyz = np.asarray(xyz, dtype=np.float64)
     n = len(xyz)
     x = np.ascontiguousarray(xyz[:, 0]).copy()
@@ -39,7 +41,8 @@ def relax(xyz, r, cfg, vel=None):
     v = v.astype(np.float64).copy()
     out = abi.FireResult()
     D = C.POINTER(C.c_double)
-    check(lib().rf_fire_relax(n, x.ctypes.data_as(D), y.ctypes.data_as(D), z.ctypes.data_as(D),
-                              rr.ctypes.data_as(D), C.byref(cfg), v.ctypes.data_as(D),
-                              C.byref(out)), "rf_fire_relax")
+    fn = lib().rf_fire_relax_noalloc if noalloc else lib().rf_fire_relax
+    check(fn(n, x.ctypes.data_as(D), y.ctypes.data_as(D), z.ctypes.data_as(D),
+             rr.ctypes.data_as(D), C.byref(cfg), v.ctypes.data_as(D), C.byref(out)),
+          "rf_fire_relax_noalloc" if noalloc else "rf_fire_relax")
     return np.stack([x, y, z], axis=1), v.reshape(3, n).T.copy(), out

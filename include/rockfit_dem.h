@@ -505,6 +505,21 @@ rf_status rf_fire_defaults(rf_fire_cfg* cfg);
 rf_status rf_fire_relax(int64_t n, double* x, double* y, double* z, const double* r,
                         const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out);
 
+/* rf_fire_relax without any allocation: same arguments (host arrays in and
+ * out) and the same results bit for bit, but every piece of device working
+ * memory -- particle state, neighbour lists, cell tables, partials, the FIRE
+ * control block and the staging of the host copies -- lives in one static
+ * device array of 64 MiB per GPU, held until the process ends; no hipMalloc,
+ * hipFree, stream creation or host allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (serving loops and
+ * harnesses that forbid device allocation).  Limit: RF_ERR_ARG when the run
+ * needs more than the array (about 140,000 particles at packing grids).
+ * Runs on the null stream and waits for it.  Every call on a GPU uses that
+ * GPU's one array: calls from several threads of the process are serialized
+ * by a lock, and nothing else may use the array. */
+rf_status rf_fire_relax_noalloc(int64_t n, double* x, double* y, double* z, const double* r,
+                                const rf_fire_cfg* cfg, double* velocity, rf_fire_result* out);
+
 #ifdef __cplusplus
 }
 #endif
