@@ -252,6 +252,33 @@ def grad_bench(steps=2000, n=20000):
     # first platen pause (the window the tangents and differences share)
     win = slice(0, npts)
     mv, mjac = autograd.macro_from_curve(e0[win], s0[win], J[win])
+    # the frozen-branch model: windows that replay the base window's discrete
+    # decisions (rf_sens_branches); its central differences are what the
+    # tangents differentiate, over the whole window, peak held in place
+    fs0, fJ, fe0, rec = autograd.run_frozen(setup, names, theta)
+    fsa = 0.5 * (np.abs(fs0[:, 0]) + np.abs(fs0[:, 1]))
+    fpeak = int(np.argmax(fsa))
+    fv, fjac = autograd.macro_from_curve(fe0, fs0, fJ, peak=fpeak)
+    frozen = {"points": int(len(fs0)), "mean_axial_stress_mpa": fv[0] * 1e3,
+              "secant_modulus_gpa": fv[1], "params": {}}
+    for k, name in enumerate(names):
+        entry = {"tangent": [fjac[0, k] * 1e3, fjac[1, k]]}
+        for rel in (1e-6, 1e-4, 1e-2):
+            h = rel * abs(theta[k])
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += h
+            tm[k] -= h
+            outs = []
+            for th_ in (tp, tm):
+                a, _, ea, ca = autograd.run_frozen(setup, names, th_, recorded=rec)
+                ca.close()
+                outs.append(autograd.macro_from_curve(ea, a, peak=fpeak)[0])
+            dm = (outs[0] - outs[1]) / (2 * h)
+            entry[f"frozen_central_difference_{rel:g}"] = [dm[0] * 1e3, dm[1]]
+            entry[f"frozen_rel_err_{rel:g}"] = [float(abs(dm[0] / fjac[0, k] - 1.0)),
+                                                float(abs(dm[1] / fjac[1, k] - 1.0))]
+        frozen["params"][name] = entry
+    rec.close()
     errs = {}
     macro = {"points": npts, "mean_axial_stress_mpa": mv[0] * 1e3, "secant_modulus_gpa": mv[1],
              "params": {}}
@@ -287,7 +314,8 @@ def grad_bench(steps=2000, n=20000):
                    "directions": names, "forward_ms": fwd * 1e3, "gradient_ms": grad * 1e3,
                    "fd_points": npts - 1,
                    "tangent_vs_central_difference_1e-2": errs,
-                   "macro_gradient": macro},
+                   "macro_gradient": macro,
+                   "frozen_branch_macro_gradient": frozen},
     }
     print(json.dumps(out))
 

@@ -136,6 +136,40 @@ struct SensState {
     double* tpart_t;
     double* tcurve;   // [point][k][bottom, top] stress tangents
     double dmat[kSensK][11];
+    // Frozen-branch windows: br_mode 1 records every discrete decision of
+    // the window, 2 replays a recorded run's decisions instead of taking
+    // them (0: decide).  Per step s (from the window's first step br_step0)
+    // and particle i, br_bits[(s*n + i)*kBrWords + w] holds: w 0 the contact
+    // slots that touch, 1 the contact slots that slip, 2 the contact slots
+    // whose rolling torque chatters, 3 the bond slots that break, 4 the
+    // particle flags kBf*; br_w[s*n + i] the spin a chattering rolling torque
+    // holds fixed; br_vw[s] the platen velocity the controller set.
+    int32_t br_mode;
+    int32_t pad2_;
+    int64_t br_step0, br_steps;
+    uint32_t* br_bits;
+    double4* br_w;
+    double* br_vw;
+};
+
+constexpr int kBrWords = 5;
+// particle flags (word 4 of a frozen-branch record)
+constexpr uint32_t kBfRoll = 1u;        // |W|^2 > 1e-18
+constexpr uint32_t kBfScope = 2u;       // inside the platens' scope
+constexpr uint32_t kBfTop = 4u;         // touches the top platen
+constexpr uint32_t kBfTopSlip = 8u;
+constexpr uint32_t kBfTopChat = 16u;
+constexpr uint32_t kBfBot = 32u;        // touches the bottom platen
+constexpr uint32_t kBfBotSlip = 64u;
+constexpr uint32_t kBfBotChat = 128u;
+
+// The platen velocity of a frozen-branch window: recorded (mode 1) or
+// replayed (mode 2) by the controller.
+struct VwLog {
+    int32_t mode;
+    int32_t pad_;
+    int64_t step0, steps;
+    double* vw;
 };
 
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,

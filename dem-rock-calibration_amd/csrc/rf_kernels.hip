@@ -762,7 +762,7 @@ __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step_batch(
 // Platen-force sum: kCtrlThreads strided running sums of the block partials,
 // then a kCtrlThreads-wide tree (the CPU restatement uses the same order).
 __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
-                                          Ctrl* __restrict__ ctrl) {
+                                          Ctrl* __restrict__ ctrl, const VwLog* lg = nullptr) {
     __shared__ double sb[kCtrlThreads];
     __shared__ double stt[kCtrlThreads];
     __shared__ long long sn[kCtrlThreads];
@@ -853,6 +853,15 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
         } else {
             c.vw = p.v_load;
         }
+        if (lg && lg->mode) {  // frozen-branch window: the velocity recorded / replayed
+            const int64_t ls = c.step - lg->step0;
+            if (ls >= 0 && ls < lg->steps) {
+                if (lg->mode == 1)
+                    lg->vw[ls] = c.vw;
+                else
+                    c.vw = lg->vw[ls];
+            }
+        }
         c.size_axis += c.vw * p.dt;
         c.top += 0.5 * c.vw * p.dt;
         c.bottom -= 0.5 * c.vw * p.dt;
@@ -924,7 +933,8 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller_sens(DevState s, De
                                                                   SensState t,
                                                                   Ctrl* __restrict__ ctrl) {
     sens_ctrl_body(p, t, ctrl);
-    ctrl_body(s, p, ctrl);
+    const VwLog lg{t.br_mode, 0, t.br_step0, t.br_steps, t.br_vw};
+    ctrl_body(s, p, ctrl, &lg);
 }
 
 // One block per sample.

@@ -177,6 +177,23 @@ struct Vec4 {
     T x, y, z, w;
 };
 
+// Discrete decisions of the pair physics, by bit: a bond breaks, a contact
+// slips (Coulomb cap), rolling friction is in its chatter regime, a particle
+// spins at all (|W|^2 > 1e-18).  The plain step takes them (Decide); a
+// frozen-branch sensitivity run takes and reports them, or replays those of
+// another run, with the spin that a chattering rolling torque holds fixed.
+constexpr int kBrBreak = 0, kBrSlip = 1, kBrChat = 2, kBrRoll = 3;
+
+struct Decide {
+    __device__ __forceinline__ bool decide(int, bool v) { return v; }
+    template <class T>
+    __device__ __forceinline__ void spin(const Vec4<T>& w, T& x, T& y, T& z) {
+        x = frozen(w.x);
+        y = frozen(w.y);
+        z = frozen(w.z);
+    }
+};
+
 __device__ __forceinline__ Vec4<double> v4(const double4& a) { return {a.x, a.y, a.z, a.w}; }
 
 // Material constants gradients flow through (DevParams holds their values).
@@ -195,13 +212,14 @@ __device__ __forceinline__ MatT<double> mat_of(const DevParams& p) {
 // stored history hist = (prev n, o_t, m_n, m_t) into the current frame,
 // increments it, and returns the force and moment contributions c, the new
 // history h (same order) and whether the bond breaks.
-template <class T>
+template <class T, class BR = Decide>
 __device__ __forceinline__ bool bond_eval_t(const MatT<T>& m, const DevParams& p,
                                             const Vec4<T>& pi, const Vec4<T>& vi,
                                             const Vec4<T>& wi, const Vec4<T>& pj,
                                             const Vec4<T>& vj, const Vec4<T>& wj,
                                             const T (&hist)[12], double lam, double L0,
-                                            double bsc, double btc, T (&c)[6], T (&h)[12]) {
+                                            double bsc, double btc, T (&c)[6], T (&h)[12],
+                                            BR&& br = BR()) {
     const T ri = pi.w, rj = pj.w;
     const T rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
     const T rijm = sqrt(rx * rx + ry * ry + rz * rz);
@@ -278,6 +296,7 @@ __device__ __forceinline__ bool bond_eval_t(const MatT<T>& m, const DevParams& p
         Mc += sqrt(mn2) * 2.0 * bb;
         brk = val(Sc) > bsc || val(Mc) > btc;
     }
+    brk = br.decide(kBrBreak, brk);
     h[0] = nx; h[1] = ny; h[2] = nz;
     h[3] = ox; h[4] = oy; h[5] = oz;
     h[6] = mnx; h[7] = mny; h[8] = mnz;
@@ -290,13 +309,14 @@ __device__ __forceinline__ bool bond_eval_t(const MatT<T>& m, const DevParams& p
 // dd_*ppHM_2, md_calculateforcesdem3.cu:202-425) for a pair known to overlap
 // (rijm < ri + rj).  Returns the force and moment contributions and the new
 // tangential overlap.
-template <class T>
+template <class T, class BR = Decide>
 __device__ __forceinline__ void contact_eval_t(const MatT<T>& m, const DevParams& p,
                                                const Vec4<T>& pi, const Vec4<T>& vi,
                                                const Vec4<T>& wi, const Vec4<T>& pj,
                                                const Vec4<T>& vj, const Vec4<T>& wj,
                                                const T (&oprev)[3], const T& wim, bool roll,
-                                               double dtI, T (&c)[6], T (&onew)[3]) {
+                                               double dtI, T (&c)[6], T (&onew)[3],
+                                               BR&& br = BR()) {
     const T ri = pi.w, mi = vi.w, rj = pj.w, mj = vj.w;
     const T rx = pj.x - pi.x, ry = pj.y - pi.y, rz = pj.z - pi.z;
     const T rijm = sqrt(rx * rx + ry * ry + rz * rz);
@@ -331,7 +351,7 @@ __device__ __forceinline__ void contact_eval_t(const MatT<T>& m, const DevParams
     ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
     T ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
     const T ftm = ftx * ftx + fty * fty + ftz * ftz;
-    if (val(ftm) > val(m.mu * m.mu * fn * fn)) {
+    if (br.decide(kBrSlip, val(ftm) > val(m.mu * m.mu * fn * fn))) {
         c1 = m.mu * fabs(fn) * (1.0 / sqrt(ftm));
         ftx *= c1; fty *= c1; ftz *= c1;
         c1 = 1.0 / Kt;
@@ -348,11 +368,14 @@ __device__ __forceinline__ void contact_eval_t(const MatT<T>& m, const DevParams
     const T tqy = (nz * ftx - nx * ftz) * ri;
     const T tqz = (nx * fty - ny * ftx) * ri;
     if (roll) {
-        if (rolling_chatter(m.mur, fn, wim, dtI)) {
-            const T cr = -m.mur * fabs(fn) * (1.0 / sqrt(frozen(wim)));
-            c[3] = tqx + frozen(wi.x) * cr;
-            c[4] = tqy + frozen(wi.y) * cr;
-            c[5] = tqz + frozen(wi.z) * cr;
+        if (br.decide(kBrChat, rolling_chatter(m.mur, fn, wim, dtI))) {
+            T fx, fy, fz;
+            br.spin(wi, fx, fy, fz);
+            const T fw = fx * fx + fy * fy + fz * fz;
+            const T cr = -m.mur * fabs(fn) * (1.0 / sqrt(fw));
+            c[3] = tqx + fx * cr;
+            c[4] = tqy + fy * cr;
+            c[5] = tqz + fz * cr;
         } else {
             const T cr = -m.mur * fabs(fn) * (1.0 / sqrt(wim));
             c[3] = tqx + wi.x * cr; c[4] = tqy + wi.y * cr; c[5] = tqz + wi.z * cr;
@@ -372,13 +395,13 @@ struct WallOutT {
 // dd_Calculate_fijtmijt_pwHM_2 (md_compression_hm.cu:20-154): R* = r_i,
 // m* = m_i, previous tangential overlap (opx, opy, opz) including the
 // reference's o.x*o.y rescale typo (:91).
-template <class T>
+template <class T, class BR = Decide>
 __device__ __forceinline__ void wall_contact_t(const MatT<T>& m, const DevParams& p, const T& ri,
                                                const T& mi, const Vec4<T>& vi, const Vec4<T>& wi,
                                                double nwx, double nwy, double nwz, double wvx,
                                                double wvy, double wvz, const T& d, const T& opx,
                                                const T& opy, const T& opz, double dtI,
-                                               WallOutT<T>& o) {
+                                               WallOutT<T>& o, BR&& br = BR()) {
     T c1 = sqrt(ri * fabs(ri - d));
     const T Kn = 2.0 * m.Ew * c1;
     const T Kt = 8.0 * m.Gw * c1;
@@ -402,7 +425,7 @@ __device__ __forceinline__ void wall_contact_t(const MatT<T>& m, const DevParams
     ox += vtx * p.dt; oy += vty * p.dt; oz += vtz * p.dt;
     T ftx = ox * Kt, fty = oy * Kt, ftz = oz * Kt;
     const T ftm = ftx * ftx + fty * fty + ftz * ftz;
-    if (val(ftm) > val(m.mu * m.mu * fnm * fnm)) {
+    if (br.decide(kBrSlip, val(ftm) > val(m.mu * m.mu * fnm * fnm))) {
         c1 = m.mu * fabs(fnm) * (1.0 / sqrt(ftm));
         ftx *= c1; fty *= c1; ftz *= c1;
         c1 = 1.0 / Kt;
@@ -417,10 +440,13 @@ __device__ __forceinline__ void wall_contact_t(const MatT<T>& m, const DevParams
     o.mty = (nwz * ftx - nwx * ftz) * ri;
     o.mtz = (nwx * fty - nwy * ftx) * ri;
     const T wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
-    if (val(wim) > 1e-18) {
-        if (rolling_chatter(m.mur, fnm, wim, dtI)) {
-            c1 = -m.mur * fabs(fnm) * (1.0 / sqrt(frozen(wim)));
-            o.max_ = frozen(wi.x) * c1; o.may = frozen(wi.y) * c1; o.maz = frozen(wi.z) * c1;
+    if (br.decide(kBrRoll, val(wim) > 1e-18)) {
+        if (br.decide(kBrChat, rolling_chatter(m.mur, fnm, wim, dtI))) {
+            T fx, fy, fz;
+            br.spin(wi, fx, fy, fz);
+            const T fw = fx * fx + fy * fy + fz * fz;
+            c1 = -m.mur * fabs(fnm) * (1.0 / sqrt(fw));
+            o.max_ = fx * c1; o.may = fy * c1; o.maz = fz * c1;
         } else {
             c1 = -m.mur * fabs(fnm) * (1.0 / sqrt(wim));
             o.max_ = wi.x * c1; o.may = wi.y * c1; o.maz = wi.z * c1;

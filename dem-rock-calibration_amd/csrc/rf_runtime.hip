@@ -130,6 +130,12 @@ struct rf_ctx {
     // Storage order: orig[p] is the caller's id of stored particle p, inv its
     // inverse (RF_ORDER_SPATIAL sorts by space at creation).
     std::vector<int32_t> orig, inv;
+    // frozen-branch record of this context (rf_sens_branches)
+    uint32_t* br_bits = nullptr;
+    double4* br_w = nullptr;
+    double* br_vw = nullptr;
+    int64_t br_cap = 0, br_recorded = 0;
+    bool br_own = false;
 
     template <class T>
     rf_status alloc(T** out, size_t count) {
@@ -1750,6 +1756,23 @@ rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
     if (!c || !params || nparam < 1 || nparam > rf::kSensK)
         return fail(RF_ERR_ARG, "rf_sens_begin: 1..%d parameters", rf::kSensK);
     if (!c->built) return fail(RF_ERR_STATE, "rf_sens_begin before rf_build_bonds");
+    // A derived time step (rf_material.dt <= 0) follows the parameters of
+    // the time scale that sets it (rf_derive): the Rayleigh time moves with
+    // E_contact and nu, the bond time with E_bond and the bond radius.  The
+    // tangents hold dt constant, so such a direction is refused.
+    if (!(c->mat.dt > 0.0)) {
+        const bool rayleigh = c->der.T_rayleigh < c->der.T_bond;
+        for (int k = 0; k < nparam; ++k) {
+            const int32_t q = params[k];
+            const bool moves = rayleigh ? (q == RF_PARAM_E_CONTACT || q == RF_PARAM_NU)
+                                        : (q == RF_PARAM_E_BOND || q == RF_PARAM_BOND_RADIUS_RATIO);
+            if (moves)
+                return fail(RF_ERR_STATE,
+                            "rf_sens_begin: dt is derived from the material (rf_material.dt <= 0) "
+                            "and moves with parameter %d; set rf_material.dt to hold it fixed",
+                            (int)q);
+        }
+    }
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     rf::SensState& t = c->sens;
@@ -1797,10 +1820,63 @@ rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
         }
     }
     t.nl = nparam;
+    t.br_step0 = c->hctrl.step;
     c->sens_np = nparam;
     c->sens_curve0 = c->hctrl.stepsave;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return RF_OK;
+}
+
+rf_status rf_sens_branches(rf_ctx* c, int32_t mode, int64_t steps, const rf_ctx* recorded) {
+    if (!c) return fail(RF_ERR_ARG, "null context");
+    rf::SensState& t = c->sens;
+    if (mode == RF_BRANCH_DECIDE) {
+        t.br_mode = 0;
+        return RF_OK;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    if (mode == RF_BRANCH_RECORD) {
+        if (steps <= 0) return fail(RF_ERR_ARG, "rf_sens_branches: steps must be > 0");
+        if (!c->br_own || c->br_cap < steps) {
+            for (void* q : {(void*)c->br_bits, (void*)c->br_w, (void*)c->br_vw}) {
+                if (!q) continue;
+                hipFree(q);
+                c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), q), c->allocs.end());
+            }
+            c->br_bits = nullptr;
+            c->br_w = nullptr;
+            c->br_vw = nullptr;
+            rf_status st = c->alloc(&c->br_bits, (size_t)steps * c->n * rf::kBrWords);
+            if (!st) st = c->alloc(&c->br_w, (size_t)steps * c->n);
+            if (!st) st = c->alloc(&c->br_vw, (size_t)steps);
+            if (st) return st;
+            c->br_cap = steps;
+            c->br_own = true;
+        }
+        t.br_mode = 1;
+        t.br_steps = steps;
+        t.br_bits = c->br_bits;
+        t.br_w = c->br_w;
+        t.br_vw = c->br_vw;
+        c->br_recorded = 0;
+        return RF_OK;
+    }
+    if (mode == RF_BRANCH_REPLAY) {
+        if (!recorded || recorded == c)
+            return fail(RF_ERR_ARG, "rf_sens_branches: replay needs another, recorded context");
+        if (recorded->n != c->n || recorded->S != c->S || recorded->device != c->device ||
+            recorded->orig != c->orig)
+            return fail(RF_ERR_ARG, "rf_sens_branches: the recorded context holds another sample");
+        if (!recorded->br_own || recorded->br_recorded <= 0)
+            return fail(RF_ERR_STATE, "rf_sens_branches: the context recorded no window");
+        t.br_mode = 2;
+        t.br_steps = recorded->br_recorded;
+        t.br_bits = recorded->br_bits;
+        t.br_w = recorded->br_w;
+        t.br_vw = recorded->br_vw;
+        return RF_OK;
+    }
+    return fail(RF_ERR_ARG, "rf_sens_branches: unknown mode %d", mode);
 }
 
 rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
@@ -1810,6 +1886,10 @@ rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
     rf_status st = sync_ctrl(c);
     if (st) return st;
     const int64_t start = c->hctrl.step;
+    const rf::SensState& tb = c->sens;
+    if (tb.br_mode && start + nsteps > tb.br_step0 + tb.br_steps)
+        return fail(RF_ERR_ARG, "rf_sens_step: the window would run past the %lld steps of its "
+                                "frozen-branch record", (long long)tb.br_steps);
     int64_t remaining = nsteps;
     while (remaining > 0 && !c->hctrl.done) {
         const int64_t k = std::min<int64_t>(remaining, c->chunk_eff);
@@ -1836,6 +1916,8 @@ rf_status rf_sens_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         }
     }
     if (done_steps) *done_steps = c->hctrl.step - start;
+    if (c->sens.br_mode == 1)
+        c->br_recorded = std::max<int64_t>(c->br_recorded, c->hctrl.step - c->sens.br_step0);
     return RF_OK;
 }
 

@@ -62,6 +62,31 @@ __device__ __forceinline__ Vec4<D> load4(const double4& v, const double4* t, int
     return r;
 }
 
+// Decisions of one pair evaluation in a frozen-branch window: taken
+// (MODE 0), taken and reported in `out` (1), or read from `in` (2), with the
+// recorded spin held by a chattering rolling torque.
+template <int MODE>
+struct SensBr {
+    uint32_t in = 0u, out = 0u;
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    __device__ __forceinline__ bool decide(int b, bool v) {
+        const bool d = MODE == 2 ? ((in >> b) & 1u) != 0u : v;
+        if (MODE == 1) out |= (uint32_t)d << b;
+        return d;
+    }
+    __device__ __forceinline__ void spin(const Vec4<D>& w, D& x, D& y, D& z) {
+        if (MODE == 2) {
+            x = dconst<1>(wx);
+            y = dconst<1>(wy);
+            z = dconst<1>(wz);
+        } else {
+            x = frozen(w.x);
+            y = frozen(w.y);
+            z = frozen(w.z);
+        }
+    }
+};
+
 __device__ __forceinline__ MatT<D> mat_dual(const DevParams& p, const SensState& t, int dir) {
     MatT<D> m;
     const double v[11] = {p.E, p.G, p.hmE, p.hmG, p.mu, p.mur, p.pA, p.Ew, p.Gw, p.b_ratio, p.b_abs};
@@ -73,9 +98,16 @@ __device__ __forceinline__ MatT<D> mat_dual(const DevParams& p, const SensState&
 
 }  // namespace
 
+template <int MODE>
 __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevState s, DevParams p, SensState t,
                                                           Ctrl* __restrict__ ctrl) {
     if (ctrl->halt) return;
+    // frozen-branch record of this step and particle (MODE 1 writes it, 2 reads it)
+    const int64_t ls = ctrl->step - t.br_step0;
+    if (MODE != 0 && (ls < 0 || ls >= t.br_steps)) {
+        if (threadIdx.x == 0) atomicOr(&ctrl->err, kErrSensList);
+        return;
+    }
     const int nl = t.nl;
     const int ppw = 64 / nl;  // particles per wave
     const int lane = threadIdx.x & 63;
@@ -98,10 +130,19 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
         const Vec4<D> wi = load4(angv[i], t.tang[cur], i, dir);
         const D ri = pi.w, mi = vi.w;
         uint32_t occ = s.occ[i], bnd = s.bnd[i], otnz = s.otnz[i];
-        // The slot table is held fixed.  Pairs that cross acut (> r_i + r_j)
-        // carry no force, so leaving them in or out changes nothing; the
-        // plain step catches up on them afterwards.  A pair that touches
-        // without a slot, or coincides, ends the window.  The particle's
+        uint32_t* const lw = MODE != 0 ? t.br_bits + (ls * p.n + i) * kBrWords : nullptr;
+        uint32_t rec[kBrWords] = {0u, 0u, 0u, 0u, 0u};  // the decisions (taken or replayed)
+        double4 wrec = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (MODE == 2) {
+#pragma unroll
+            for (int u = 0; u < kBrWords; ++u) rec[u] = lw[u];
+            wrec = t.br_w[ls * p.n + i];
+        }
+        // The slot table is held fixed.  Contact pairs that cross acut
+        // (> r_i + r_j) carry no force, so leaving them in or out changes
+        // nothing; the plain step catches up on them afterwards.  A pair that
+        // touches without a slot, or coincides, or a bond stretched past acut
+        // (which the plain step would drop without a break) ends the window.  The particle's
         // lanes share the checks (blocks of 8 slots or candidates, every
         // gather of a block issued at once) and also find which slots
         // overlap, for the contact sums below.
@@ -125,7 +166,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 if (!((blk >> u) & 1u)) continue;
                 const double dx = q[u].x - pi.x.v, dy = q[u].y - pi.y.v, dz = q[u].z - pi.z.v;
                 const double rr = dx * dx + dy * dy + dz * dz;
-                bad = bad || rr < 1e-10;
+                bad = bad || rr < 1e-10 || (((bnd >> (8 * j + u)) & 1u) && rr > p.acut2);
                 if (::sqrt(rr) < pi.w.v + q[u].w) ovl_l |= 1u << (8 * j + u);
             }
         }
@@ -229,8 +270,11 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 #pragma unroll
                 for (int u = 0; u < 12; ++u) hist[u] = mk(hv[u], tb[u]);
                 D c6[6], h[12];
+                SensBr<MODE> br;
+                br.in = ((rec[3] >> ks) & 1u) << kBrBreak;
                 const bool brk = bond_eval_t<D>(m, p, pi, vi, wi, pj, vj, wj, hist, r.invL0, r.L0,
-                                                r.sc, r.tc, c6, h);
+                                                r.sc, r.tc, c6, h, br);
+                if (MODE == 1) rec[3] |= (uint32_t)brk << ks;
 #pragma unroll
                 for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c6[u];
                 if (brk) {
@@ -252,9 +296,15 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
         tree_sum(bond_lane, bsum);
         // contacts in slot order; slot k goes to lane k % 8
         const D wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
-        const bool roll = wim.v > 1e-18;
+        const bool roll = MODE == 2 ? (rec[4] & kBfRoll) != 0u : wim.v > 1e-18;
         const uint32_t cmask = occ & ~bnd;
-        const uint32_t ovl = cmask & ovl_all;  // contacts that touch
+        // contacts that touch
+        const uint32_t ovl = cmask & (MODE == 2 ? rec[0] : ovl_all);
+        if (MODE == 1) {
+            rec[0] = ovl;
+            rec[4] |= roll ? kBfRoll : 0u;
+            wrec = make_double4(wi.x.v, wi.y.v, wi.z.v, 0.0);
+        }
         auto contact_lane = [&](int l, D (&acc)[6]) {
 #pragma unroll 1
             for (int k = l; k < 32; k += kLanes) {
@@ -271,7 +321,17 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 D op[3] = {mk(has ? ov.x : 0.0, has ? tq.x : 0.0), mk(has ? ov.y : 0.0, has ? tq.y : 0.0),
                            mk(has ? ov.z : 0.0, has ? tq.z : 0.0)};
                 D c6[6], on[3];
-                contact_eval_t<D>(m, p, pi, vi, wi, pj, vj, wj, op, wim, roll, s.pk[i].y, c6, on);
+                SensBr<MODE> br;
+                br.in = (((rec[1] >> k) & 1u) << kBrSlip) | (((rec[2] >> k) & 1u) << kBrChat);
+                br.wx = wrec.x;
+                br.wy = wrec.y;
+                br.wz = wrec.z;
+                contact_eval_t<D>(m, p, pi, vi, wi, pj, vj, wj, op, wim, roll, s.pk[i].y, c6, on,
+                                  br);
+                if (MODE == 1) {
+                    rec[1] |= ((br.out >> kBrSlip) & 1u) << k;
+                    rec[2] |= ((br.out >> kBrChat) & 1u) << k;
+                }
                 if (lead) s.ot[qs] = make_double4(on[0].v, on[1].v, on[2].v, 0.0);
                 t.tot[qs * kSensK + dir] = make_double4(on[0].d[0], on[1].d[0], on[2].d[0], 0.0);
 #pragma unroll
@@ -305,26 +365,49 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 r2 = px_ * px_ + py_ * py_;
             }
             D s_b = dconst<1>(0.0), s_t = dconst<1>(0.0);
-            if (r2.v < p.rr_scope && pz_.v < p.zcut) {
+            const bool scope = MODE == 2 ? (rec[4] & kBfScope) != 0u
+                                         : (r2.v < p.rr_scope && pz_.v < p.zcut);
+            if (MODE == 1 && scope) rec[4] |= kBfScope;
+            if (scope) {
                 const int ax = p.axis;
                 const D ppos = ax == 2 ? pz_ : py_;
                 D o0 = dconst<1>(0.0), o1 = o0, o2 = o0;
                 WallOutT<D> wo;
-                if (top - ppos.v < ri.v) {
+                SensBr<MODE> br;
+                br.wx = wrec.x;
+                br.wy = wrec.y;
+                br.wz = wrec.z;
+                const bool t_on = MODE == 2 ? (rec[4] & kBfTop) != 0u : top - ppos.v < ri.v;
+                if (t_on) {
+                    br.in = (((rec[4] & kBfTopSlip) ? 1u : 0u) << kBrSlip) |
+                            (((rec[4] & kBfTopChat) ? 1u : 0u) << kBrChat) |
+                            (((rec[4] & kBfRoll) ? 1u : 0u) << kBrRoll);
+                    br.out = 0u;
                     const double nwy = ax == 1 ? 1.0 : 0.0, nwz = ax == 2 ? 1.0 : 0.0;
                     const double wvy = ax == 1 ? vw : 0.0, wvz = ax == 2 ? vw : 0.0;
                     wall_contact_t<D>(m, p, ri, mi, vi, wi, 0.0, nwy, nwz, 0.0, wvy, wvz,
-                                      fabs(top - ppos), o0, o1, o2, s.pk[i].y, wo);
+                                      fabs(top - ppos), o0, o1, o2, s.pk[i].y, wo, br);
+                    if (MODE == 1)
+                        rec[4] |= kBfTop | ((br.out >> kBrSlip) & 1u ? kBfTopSlip : 0u) |
+                                  ((br.out >> kBrChat) & 1u ? kBfTopChat : 0u);
                     s_t += ax == 2 ? wo.fnz : wo.fny;
                     F[0] += wo.fnx + wo.ftx; F[1] += wo.fny + wo.fty; F[2] += wo.fnz + wo.ftz;
                     M[0] += wo.mtx + wo.max_; M[1] += wo.mty + wo.may; M[2] += wo.mtz + wo.maz;
                     o0 = wo.ox; o1 = wo.oy; o2 = wo.oz;
                 }
-                if (ppos.v - bottom < ri.v) {
+                const bool b_on = MODE == 2 ? (rec[4] & kBfBot) != 0u : ppos.v - bottom < ri.v;
+                if (b_on) {
+                    br.in = (((rec[4] & kBfBotSlip) ? 1u : 0u) << kBrSlip) |
+                            (((rec[4] & kBfBotChat) ? 1u : 0u) << kBrChat) |
+                            (((rec[4] & kBfRoll) ? 1u : 0u) << kBrRoll);
+                    br.out = 0u;
                     const double nwy = ax == 1 ? -1.0 : 0.0, nwz = ax == 2 ? -1.0 : 0.0;
                     const double wvy = ax == 1 ? -vw : 0.0, wvz = ax == 2 ? -vw : 0.0;
                     wall_contact_t<D>(m, p, ri, mi, vi, wi, 0.0, nwy, nwz, 0.0, wvy, wvz,
-                                      fabs(bottom - ppos), o0, o1, o2, s.pk[i].y, wo);
+                                      fabs(bottom - ppos), o0, o1, o2, s.pk[i].y, wo, br);
+                    if (MODE == 1)
+                        rec[4] |= kBfBot | ((br.out >> kBrSlip) & 1u ? kBfBotSlip : 0u) |
+                                  ((br.out >> kBrChat) & 1u ? kBfBotChat : 0u);
                     s_b += ax == 2 ? wo.fnz : wo.fny;
                     F[0] += wo.fnx + wo.ftx; F[1] += wo.fny + wo.fty; F[2] += wo.fnz + wo.ftz;
                     M[0] += wo.mtx + wo.max_; M[1] += wo.mty + wo.may; M[2] += wo.mtz + wo.maz;
@@ -356,6 +439,11 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
             s.occ[i] = occ;
             s.bnd[i] = bnd;
             s.otnz[i] = otnz;
+            if (MODE == 1) {
+#pragma unroll
+                for (int u = 0; u < kBrWords; ++u) lw[u] = rec[u];
+                t.br_w[ls * p.n + i] = wrec;
+            }
             const double4 rb = s.rbuild[i];
             const double ex = nrx.v - rb.x, ey = nry.v - rb.y, ez = nrz.v - rb.z;
             if (ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
@@ -419,8 +507,13 @@ hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensSta
                             hipStream_t st) {
     if (p.B != kMaxBonds) return hipErrorInvalidValue;  // bond rows read 16 entries at once
     const int64_t per_block = (int64_t)(kSensBlock / 64) * (64 / t.nl);
-    hipLaunchKernelGGL(k_sens_step, dim3((unsigned)((p.n + per_block - 1) / per_block)),
-                       dim3(kSensBlock), 0, st, s, p, t, ctrl);
+    const dim3 grid((unsigned)((p.n + per_block - 1) / per_block));
+    if (t.br_mode == 1)
+        hipLaunchKernelGGL(k_sens_step<1>, grid, dim3(kSensBlock), 0, st, s, p, t, ctrl);
+    else if (t.br_mode == 2)
+        hipLaunchKernelGGL(k_sens_step<2>, grid, dim3(kSensBlock), 0, st, s, p, t, ctrl);
+    else
+        hipLaunchKernelGGL(k_sens_step<0>, grid, dim3(kSensBlock), 0, st, s, p, t, ctrl);
     hipLaunchKernelGGL(k_sens_partials, dim3((unsigned)((p.nblk + 7) / 8)), dim3(256), 0, st, p, t,
                        s, ctrl);
     return hipGetLastError();

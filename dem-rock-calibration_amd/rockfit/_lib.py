@@ -31,7 +31,7 @@ EXPORTS = [
     "rf_batch_create", "rf_batch_destroy", "rf_batch_step", "rf_batch_run", "rf_batch_last_ms",
     "rf_linearity_deviation", "rf_curve_moduli", "rf_batch_tuning", "rf_run_uniaxial",
     "rf_run_brazilian", "rf_adjoint_run", "rf_batch_set_form", "rf_capture_chunk",
-    "rf_fire_relax_noalloc",
+    "rf_fire_relax_noalloc", "rf_sens_branches",
 ]
 
 
@@ -94,6 +94,8 @@ def lib():
     L.rf_last_timing.restype = st
     L.rf_sens_material_tangent.argtypes = [P(abi.Material), C.c_double, C.c_int32, P(C.c_double)]
     L.rf_sens_material_tangent.restype = st
+    L.rf_sens_branches.argtypes = [vp, C.c_int32, C.c_int64, vp]
+    L.rf_sens_branches.restype = st
     L.rf_sens_begin.argtypes = [vp, C.c_int32, P(C.c_int32)]
     L.rf_sens_begin.restype = st
     L.rf_sens_step.argtypes = [vp, C.c_int64, P(C.c_int64)]

@@ -26,6 +26,9 @@ RF_QUIRK_BEND_DOUBLE_SCALE = 0x1
 RF_QUIRK_ROLL_RESET = 0x2
 RF_QUIRK_SLOT0_SENTINEL = 0x4
 RF_ORDER_SPATIAL = 0
+RF_BRANCH_DECIDE = 0
+RF_BRANCH_RECORD = 1
+RF_BRANCH_REPLAY = 2
 RF_ORDER_GIVEN = 1
 RF_QUIRKS_REFERENCE = 0x7
 

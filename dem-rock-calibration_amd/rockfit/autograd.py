@@ -68,7 +68,32 @@ def run_window(setup, names, values, sens=True, strain=False):
     return curve[:, 1:3].copy(), J
 
 
-def macro_from_curve(strain, stress, J=None):
+def run_frozen(setup, names, values, recorded=None):
+    """One loading window of the frozen-branch model (rf_sens_branches).
+
+    recorded None: the window decides and records every discrete decision
+    (bond breaks, contact touch and slip, rolling and its chatter, platen
+    contacts, the platen velocity); else it replays the decisions the
+    context `recorded` recorded.  Returns (stress (points, 2), Jacobian
+    (points, nparam, 2), strain (points,), context); keep the recording
+    context alive while windows replay it."""
+    mat = with_params(setup["mat"], names, values)
+    c = Context(setup["xyz"], setup["r"], mat, setup["box"])
+    c.build_bonds()
+    c.set_test(setup["cfg"])
+    if recorded is None:
+        c.sens_branches(abi.RF_BRANCH_RECORD, setup["steps"])
+    else:
+        c.sens_branches(abi.RF_BRANCH_REPLAY, 0, recorded)
+    c.sens_begin([PARAMS[n][0] for n in names])
+    done = c.sens_step(setup["steps"])
+    if done != setup["steps"]:
+        raise RuntimeError(f"the loading window ended after {done} of {setup['steps']} steps")
+    curve = c.curve()
+    return curve[:, 1:3].copy(), c.sens_curve(), curve[:, 0].copy(), c
+
+
+def macro_from_curve(strain, stress, J=None, peak=None):
     """Mean axial stress and secant modulus of a loading window (reduced
     units: stress and modulus in GPa), and their Jacobian when the curve's
     is given.
@@ -81,10 +106,11 @@ def macro_from_curve(strain, stress, J=None):
     secant modulus is s_peak / e_peak (md_results.cu:149), the peak being the
     first point of highest axial stress (the peak moves only on a strict
     increase, md_results.cu:70-75) and e_peak = -strain there.  The peak's
-    position is an event: its derivative follows the point it picks.
+    position is an event: its derivative follows the point it picks; `peak`
+    holds it at a given point (the frozen-branch model).
     Returns (values (2,), Jacobian (2, nparam) or None)."""
     sa = 0.5 * (np.abs(stress[:, 0]) + np.abs(stress[:, 1]))
-    p = int(np.argmax(sa))
+    p = int(np.argmax(sa)) if peak is None else int(peak)
     eps = -float(strain[p])
     if not eps > 0.0:
         raise ValueError("the window's highest stress comes before any platen travel")

@@ -260,7 +260,7 @@ def batch_workloads(count=64, seed=SEED + 4, n_lo=2000, n_hi=40000):
     return out
 
 
-def gradient_workload(n=20000, steps=2000, seed=SEED + 5, dnsave=50):
+def gradient_workload(n=20000, steps=2000, seed=SEED + 5, dnsave=50, min_stress_mpa=0.1):
     """C5: a 20k-sphere cylinder (r ~ U[0.20, 0.30] mm) loaded for `steps`
     elastic pre-peak steps to 0.1 % axial strain (closing speed
     0.001 H / (steps dt)), with dt fixed at the value rf_derive gives (the
@@ -269,7 +269,7 @@ def gradient_workload(n=20000, steps=2000, seed=SEED + 5, dnsave=50):
 
     w = make_workload(n=n, r_min=0.20, r_max=0.30, test="uniaxial", seed=seed, generator="rsi",
                       test_kwargs=dict(dnsave=dnsave, max_steps=100 * steps, max_strain=0.05,
-                                       min_stress_mpa=0.1))
+                                       min_stress_mpa=min_stress_mpa))
     m = abi.Material.from_buffer_copy(w["mat"])
     m.dt = core.derive(w["mat"], float(np.min(w["r"])), float(np.max(w["r"]))).dt
     w["mat"] = m

@@ -206,6 +206,15 @@ class Context:
         check(lib().rf_sens_begin(self.h, len(params), arr), "rf_sens_begin")
         self._sens_np = len(params)
 
+    def sens_branches(self, mode, steps=0, recorded=None):
+        """Frozen-branch windows (rf_sens_branches): abi.RF_BRANCH_RECORD the
+        next windows' decisions for up to `steps` steps, RF_BRANCH_REPLAY
+        those another context recorded, RF_BRANCH_DECIDE."""
+        check(lib().rf_sens_branches(self.h, int(mode), int(steps),
+                                     recorded.h if recorded is not None else None),
+              "rf_sens_branches")
+        self._branch_src = recorded  # keep the recording alive
+
     def sens_step(self, nsteps):
         done = C.c_int64()
         check(lib().rf_sens_step(self.h, int(nsteps), C.byref(done)), "rf_sens_step")

@@ -336,8 +336,15 @@ rf_status rf_set_graph(rf_ctx* ctx, int32_t enable);
  * Between rf_sens_begin and the last rf_sens_step the context advances with
  * the sensitivity kernel (values equal the plain step's bit for bit: same
  * operations, same per-particle sum order); the slot table is held
- * (pairs crossing acut carry no force; a touching pair without a slot ends
- * the window with RF_ERR_STATE), dt is a constant of the run.
+ * (contact pairs crossing acut carry no force; a touching pair without a
+ * slot, or a bond stretched past acut, ends the window with RF_ERR_STATE),
+ * dt is a constant of the run: with a derived dt (rf_material.dt <= 0) a
+ * direction that moves the time scale setting it (rf_derive: E_contact and
+ * nu for the Rayleigh time, E_bond and the bond radius for the bond time) is
+ * refused with RF_ERR_STATE.  The state at rf_sens_begin (positions,
+ * velocities, histories) is treated as independent of the parameters: the
+ * tangents start at zero there, so a window that starts after loading began
+ * differentiates the rest of the run from that state on.
  * ------------------------------------------------------------------------ */
 #define RF_SENS_MAX 4
 
@@ -362,6 +369,26 @@ rf_status rf_sens_step(rf_ctx* ctx, int64_t nsteps, int64_t* done_steps);
 /* Tangents of the curve points recorded since rf_sens_begin:
  * out[(point*nparam + k)*2 + {0,1}] = d(bottom, top platen stress)/d(param k). */
 rf_status rf_sens_get_curve(rf_ctx* ctx, double* out, int64_t cap, int64_t* n);
+
+/* Frozen-branch windows.  A sensitivity window decides, at every step,
+ * which bonds break, which contacts touch and which slip, whether rolling
+ * friction acts and whether it chatters (then with the particle's spin held
+ * fixed, as the tangents do), which particles touch a platen and slip on it,
+ * and the platen velocity (controller).  Its tangents are the derivatives of
+ * the run with all those decisions held.  RF_BRANCH_RECORD makes the next
+ * windows of ctx (rf_sens_begin, rf_sens_step) record them, for up to `steps`
+ * steps from the window's start; RF_BRANCH_REPLAY makes the next windows of
+ * ctx take them from `recorded` (a context of the same sample on the same
+ * device that recorded a window from the same starting step; it must stay
+ * alive and record nothing else meanwhile) instead of deciding.  A replayed
+ * window at the recorded parameters repeats the recorded run bit for bit;
+ * at nearby parameters it is the frozen-branch model whose central
+ * differences the tangents equal.  RF_BRANCH_DECIDE returns to deciding.
+ * No counterpart in the reference. */
+#define RF_BRANCH_DECIDE 0
+#define RF_BRANCH_RECORD 1
+#define RF_BRANCH_REPLAY 2
+rf_status rf_sens_branches(rf_ctx* ctx, int32_t mode, int64_t steps, const rf_ctx* recorded);
 
 /* Gradient of a loading window's macroscopic response in one call:
  * rf_sens_begin(params), rf_sens_step(steps), then over the curve points the
