@@ -176,3 +176,23 @@ def test_save_results_format(checker, tmp_path):
     vals = np.array([float(v) for v in m.groups()])
     np.testing.assert_allclose(vals[:4], st[0]["F"], rtol=1e-5)
     np.testing.assert_allclose(vals[4], st[0]["F"].sum(), rtol=1e-5)
+
+
+def test_python_cg_equals_the_command_line_step(checker, tmp_path):
+    """rockfit.calibrate.ConjugateGradient (the CG step with the Jacobian
+    given) lands where cli/rf_cg.h's step lands on a linear model, whose
+    forward differences are exact: three steps, all five parameters."""
+    from rockfit.calibrate import ConjugateGradient
+
+    A, b, P0 = model(6)
+    goal = np.array([3.0, 4.0, 5.0, 6.0])
+    mult = np.array([50.0, 100.0, 1.0, 10000.0])
+    W, st, _, _ = run(checker, tmp_path, A, b, P0, goal, mult, vtype=2, steps=3)
+    cg = ConjugateGradient(goal, mult)
+    np.testing.assert_array_equal(cg.weight, W)
+    P = P0.copy()
+    for k in range(3):
+        P = cg.step(P, A @ P + b, A.T)
+        np.testing.assert_allclose(P, st[k]["params"], rtol=1e-8)
+        np.testing.assert_allclose(cg.dF, st[k]["dF"], rtol=1e-7, atol=1e-9 * np.abs(cg.dF).max())
+        np.testing.assert_allclose(cg.F, st[k]["F"], rtol=1e-8)

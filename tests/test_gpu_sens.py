@@ -171,3 +171,28 @@ def test_adjoint_run_equals_macro_from_curve(base):
     np.testing.assert_allclose(vals, v, rtol=1e-12)
     np.testing.assert_allclose(grad, jac, rtol=1e-12, atol=1e-12 * np.abs(jac).max())
     c.close()
+
+
+def test_derived_dt_refuses_the_parameters_that_move_it():
+    """With dt derived from the material (rf_material.dt <= 0), the
+    directions that move the time scale setting it are refused (the
+    tangents hold dt constant); the others are accepted."""
+    from rockfit import abi
+    from rockfit._lib import RockfitError
+    from rockfit.core import derive
+
+    w = config.make_workload(n=600, r_min=0.2, r_max=0.3, test="uniaxial", seed=3,
+                             test_kwargs=dict(v_load_mps=-2.0, dnsave=10))
+    assert not w["mat"].dt > 0.0
+    d = derive(w["mat"], float(w["r"].min()), float(w["r"].max()))
+    moving = (["E_contact", "nu"] if d.T_rayleigh < d.T_bond
+              else ["E_bond", "bond_radius_ratio"])
+    c = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    c.build_bonds()
+    c.set_test(w["cfg"])
+    for name in moving:
+        with pytest.raises(RockfitError) as e:
+            c.sens_begin([autograd.PARAMS[name][0]])
+        assert e.value.status == abi.RF_ERR_STATE
+    c.sens_begin([autograd.PARAMS["friction"][0], autograd.PARAMS["rolling_friction"][0]])
+    assert c.sens_step(5) == 5
