@@ -1,6 +1,10 @@
 """RockFit-DEM on the GPU: the command line's ResultCalculateOneSample line
 and curve files equal what the Python API computes through the same C-ABI
-on the same sample file (which the loading parity tests pin to the oracle).
+on the same sample file, and what the CPU oracle computes on that file
+under the reference's configuration (SetInitialParameters,
+mp_determineconjgradient.cu:15-47: p_r 0.5 mm, one modulus for bonds and
+contacts, acut 3.2 p_r, the DCG runs re-gridded at 2.9 p_r (:65-66), the
+reference's quirks).
 
 Small specimens (1500 spheres of radius 0.5 mm, the reference's p_r) are
 written as ./result/Sample_<k>_0.txt in a temporary working directory and
@@ -43,10 +47,11 @@ def make_sample(kind, n, seed):
     return xyz + np.array(box.center[:])
 
 
-def api_run(kind, path, values):
+def api_run(kind, path, values, oracle=False):
     """The same run through the Python API: h_LoadCylinderSample's size_real
     and center_real (md_createcylindersample.cu:448-465), driver values of
-    mp_calculateonesample.cu:120-124."""
+    mp_calculateonesample.cu:120-124.  oracle: also run it in the CPU
+    oracle and return its test state and curve."""
     xyz, _ = sample.read_sample(path)
     r = np.full(len(xyz), P_R)
     mat = config.reference_material(E=values[0], sigma_c=values[1], tau_c=values[2],
@@ -66,11 +71,31 @@ def api_run(kind, path, values):
     ts = g.run()
     curve = g.curve()
     g.close()
+    if oracle:
+        return mat, ts, curve, oracle_run(kind, path, mat, box, cfg)
     return mat, ts, curve
 
 
 def fmt(v):
     return "%g" % v
+
+
+def oracle_run(kind, path, mat, box, cfg):
+    """The whole test in the CPU oracle on the same sample file: lists,
+    bonds, loading to the end, CalculateLinearityDeviation and
+    CalculateDestruction."""
+    import pyoracle
+
+    xyz, _ = sample.read_sample(path)
+    r = np.full(len(xyz), P_R)
+    o = pyoracle.Oracle(xyz, r, mat, box, threads=16)
+    o.build_bonds()
+    o.set_test(cfg)
+    o.step(1 << 40)
+    o.post()
+    ts = o.test_state()
+    assert ts.done
+    return ts, o.curve()
 
 
 def check_curve_file(path, curve):
@@ -81,7 +106,8 @@ def check_curve_file(path, curve):
         assert line == " ".join([fmt(row[0]), fmt(row[1] * 1e9), fmt(row[2] * 1e9)])
 
 
-def test_cli_uniaxial_equals_api(binary, tmp_path):
+@pytest.mark.timeout(900)
+def test_cli_uniaxial_equals_api_and_oracle(binary, tmp_path):
     print("case cli uniaxial n=1500 seed=5", flush=True)
     os.makedirs(tmp_path / "result")
     path = str(tmp_path / "result" / "Sample_0_0.txt")
@@ -89,29 +115,38 @@ def test_cli_uniaxial_equals_api(binary, tmp_path):
     res = farm.run(MAT, "U", gpu=0, index=5, cwd=str(tmp_path), timeout=300)
     assert res["returncode"] == 0, res["stderr"][-2000:]
     assert res["index"] == 5 and res["gpu"] == 0
-    mat, ts, curve = api_run(0, path, MAT)
+    mat, ts, curve, (ots, ocurve) = api_run(0, path, MAT, oracle=True)
     assert ts.loading, "the platens never loaded the specimen"
-    want = [mat.E_bond * 1e9, mat.sigma_c * 1e9, mat.tau_c * 1e9, mat.bond_radius * 1e-3,
-            mat.bond_gap * 1e-3, MAT[5], ts.peak_strain, ts.peak_stress * 1e9,
-            ts.linearity_deviation, ts.destruction, 1e-2, 1e6, 0, 1.0]
-    assert res["result_text"] == ",".join(fmt(v) for v in want)
-    check_curve_file(str(tmp_path / "result" / "Uniaxial_0.txt"), curve)
+    for t in (ts, ots):
+        want = [mat.E_bond * 1e9, mat.sigma_c * 1e9, mat.tau_c * 1e9, mat.bond_radius * 1e-3,
+                mat.bond_gap * 1e-3, MAT[5], t.peak_strain, t.peak_stress * 1e9,
+                t.linearity_deviation, t.destruction, 1e-2, 1e6, 0, 1.0]
+        assert res["result_text"] == ",".join(fmt(v) for v in want)
+    print(f"uniaxial: {ots.step} loading steps, UCS {ots.peak_stress * 1e3:.4f} MPa, "
+          f"equal to the oracle", flush=True)
+    np.testing.assert_array_equal(curve, ocurve)
+    check_curve_file(str(tmp_path / "result" / "Uniaxial_0.txt"), ocurve)
 
 
-def test_cli_brazilian_equals_api(binary, tmp_path):
+@pytest.mark.timeout(900)
+def test_cli_brazilian_equals_api_and_oracle(binary, tmp_path):
     print("case cli brazilian n=1500 seed=6", flush=True)
     os.makedirs(tmp_path / "result")
     path = str(tmp_path / "result" / "Sample_1_0.txt")
     sample.write_sample(path, make_sample(1, 1500, 6))
     res = farm.run(MAT, "B", gpu=0, index=6, cwd=str(tmp_path), timeout=300)
     assert res["returncode"] == 0, res["stderr"][-2000:]
-    mat, ts, curve = api_run(1, path, MAT)
+    mat, ts, curve, (ots, ocurve) = api_run(1, path, MAT, oracle=True)
     assert ts.loading
-    want = [mat.E_bond * 1e9, mat.sigma_c * 1e9, mat.tau_c * 1e9, mat.bond_radius * 1e-3,
-            mat.bond_gap * 1e-3, MAT[5], 1e-2, 1e6, 1.0, 1.0, ts.peak_strain,
-            ts.peak_stress * 1e9, 0, ts.destruction]
-    assert res["result_text"] == ",".join(fmt(v) for v in want)
-    check_curve_file(str(tmp_path / "result" / "Brazil_0.txt"), curve)
+    for t in (ts, ots):
+        want = [mat.E_bond * 1e9, mat.sigma_c * 1e9, mat.tau_c * 1e9, mat.bond_radius * 1e-3,
+                mat.bond_gap * 1e-3, MAT[5], 1e-2, 1e6, 1.0, 1.0, t.peak_strain,
+                t.peak_stress * 1e9, 0, t.destruction]
+        assert res["result_text"] == ",".join(fmt(v) for v in want)
+    print(f"brazilian: {ots.step} loading steps, tensile strength {ots.peak_stress * 1e3:.4f} "
+          f"MPa, equal to the oracle", flush=True)
+    np.testing.assert_array_equal(curve, ocurve)
+    check_curve_file(str(tmp_path / "result" / "Brazil_0.txt"), ocurve)
 
 
 def cg_box(kind):
@@ -134,6 +169,7 @@ def cg_material(P):
     return m
 
 
+@pytest.mark.timeout(900)
 def test_cli_dcg_step_uses_the_right_runs(binary, tmp_path):
     """Two steps of DetermineConjGradient on a small uniaxial sample: the
     base run and the three varied runs (type 7: sigma_c, tau_c, b_r, each
@@ -163,13 +199,18 @@ def test_cli_dcg_step_uses_the_right_runs(binary, tmp_path):
         cfg.center[k] = 0.5 * (hi[k] + lo[k])
     box = cg_box(0)
 
-    def results(P):
+    def results(P, oracle=False):
         g = Context(xyz, r, cg_material(P), box)
         g.build_bonds()
         g.set_test(cfg)
         ts = g.run()
         g.close()
-        return np.array([ts.peak_strain, ts.peak_stress, 1.0, ts.linearity_deviation])
+        out = np.array([ts.peak_strain, ts.peak_stress, 1.0, ts.linearity_deviation])
+        if oracle:  # the base run of the first step, in the CPU oracle
+            ots, _ = oracle_run(0, path, cg_material(P), box, cfg)
+            np.testing.assert_array_equal(
+                out, [ots.peak_strain, ots.peak_stress, 1.0, ots.linearity_deviation])
+        return out
 
     goal = np.array([0.00480002, 104.0497482e6 / 1e9, 10.5e6 / 1e9, 1.0])
     W = np.array([50.0, 100.0, 1.0, 10000.0]) / goal ** 2
@@ -184,7 +225,7 @@ def test_cli_dcg_step_uses_the_right_runs(binary, tmp_path):
         got = np.array([float(v) for v in base[1].split(":")[1].split()])
         np.testing.assert_allclose(got, P * np.array([1e9, 1e9, 1e9, 1e-3, 1e-3]), rtol=1e-5)
         var = varied * 0.1 * P
-        R0 = results(P)
+        R0 = results(P, oracle=step == 0)
         assert R0[1] > 0.0
         want = "%g %g %g" % (R0[0], R0[1] * 1e9, R0[2] * 1e9)
         assert base[2].split(": ")[1] == want
