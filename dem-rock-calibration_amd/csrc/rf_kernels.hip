@@ -1,13 +1,16 @@
 // rf_kernels.hip -- gfx950 kernels of the bonded-particle DEM loading step.
 //
-// One 32-lane group per particle (two particles per wave64, eight per
-// 256-thread block).  In one launch a group
+// One 8-lane group per particle (eight particles per wave64, 32 per
+// 256-thread block); lane l of a group owns slots, candidates and bonds
+// l, l+8, l+16, l+24.  In one launch a group
 //   1. prunes and appends its interaction row (slot lanes check the stored
 //      partners, candidate lanes look for new ones),
-//   2. evaluates its bonds (one lane per bond) and contacts (one lane per
-//      slot) and reduces their contributions with a fixed 32-wide tree,
+//   2. evaluates its bonds and contacts (each lane its own entries, in
+//      ascending order) and reduces the 8 lane sums with a fixed tree,
 //   3. adds the platen forces, kick-drifts the particle, applies the cut
 //      restriction and writes the next state buffer.
+// Particles are stored in spatial (Morton) order (rf_create); the ids of
+// the caller rank new pairs and order the cells.
 // A one-block controller kernel then sums the platen partials, runs the
 // stress/peak/fracture logic and moves the platens.  No float atomics: every
 // sum has a fixed order, so results are bitwise reproducible and equal to the
@@ -761,66 +764,26 @@ __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step_batch(
 // CompressU_S3 / CompressB_S3 (md_data_types.h:347-379, 462-494), on device.
 // Platen-force sum: kCtrlThreads strided running sums of the block partials,
 // then a kCtrlThreads-wide tree (the CPU restatement uses the same order).
-__device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
-                                          Ctrl* __restrict__ ctrl, const VwLog* lg = nullptr) {
-    __shared__ double sb[kCtrlThreads];
-    __shared__ double stt[kCtrlThreads];
-    __shared__ long long sn[kCtrlThreads];
-    const int t = threadIdx.x;
-    // the partials are read before the halt flag is known (a halted step
-    // leaves them as they were), so these loads and the control block's
-    // (uniform: scalar loads) are in flight together; nothing in this
-    // kernel writes the control block before its last store
-    const Ctrl c0 = *ctrl;
-    double ab = 0.0, at = 0.0;
-    long long nb = 0;
-#pragma unroll 4
-    for (int b = t; b < p.nblk; b += kCtrlThreads) {
-        ab = ab + s.part_b[b];
-        at = at + s.part_t[b];
-        nb += s.part_bonds[b];
-    }
-    if (c0.halt) return;
-    // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
-    // one shuffle per level
-    sb[t] = ab;
-    stt[t] = at;
-    sn[t] = nb;
-    __syncthreads();
-    for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
-        if (t < off) {
-            sb[t] = sb[t] + sb[t + off];
-            stt[t] = stt[t] + stt[t + off];
-            sn[t] += sn[t + off];
-        }
-        __syncthreads();
-    }
-    if (t >= 64) return;
-    double vb = sb[t], vt = stt[t];
-    long long vn = sn[t];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        vb = vb + __shfl_down(vb, off);
-        vt = vt + __shfl_down(vt, off);
-        vn += __shfl_down(vn, off);
-    }
-    if (t != 0) return;
-    sb[0] = vb;
-    stt[0] = vt;
-    sn[0] = vn;
+// The controller proper, on one thread: SumForcesULoading / SumForcesBLoading
+// (md_results.cu:56-145) on the summed platen forces and bond count, the
+// curve point, and CompressU_S3 / CompressB_S3 (md_data_types.h:347-379,
+// 462-494).
+__device__ __forceinline__ void ctrl_logic(const DevState& s, const DevParams& p,
+                                           Ctrl* __restrict__ ctrl, const Ctrl& c0, double sum_b,
+                                           double sum_t, long long sum_n, const VwLog* lg) {
     Ctrl c = c0;
     if (p.kind != RF_TEST_NONE) {
-        c.sfl0 = sb[0];
-        c.sfl1 = stt[0];
+        c.sfl0 = sum_b;
+        c.sfl1 = sum_t;
         const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
         if (c.step % dn == 0) {
             if (c.stepsave < p.curve_cap) {
-                s.bcount[c.stepsave] = sn[0];
+                s.bcount[c.stepsave] = sum_n;
                 s.curve[3 * c.stepsave] = (c.size_axis - c.size0) / c.size0;
                 s.curve[3 * c.stepsave + 1] = c.sfl0 * p.inv_area;
                 s.curve[3 * c.stepsave + 2] = c.sfl1 * p.inv_area;
             }
-            c.last_bonds = sn[0];
+            c.last_bonds = sum_n;
             if (fabs(c.vw) > 1e-9) ++c.stepsave;
         }
         c.stress = 0.5 * (fabs(c.sfl0) + fabs(c.sfl1)) * p.inv_area;
@@ -876,6 +839,53 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     }
     c.halt = (c.done || c.rebuild || c.err || (c.stop_at > 0 && c.step >= c.stop_at)) ? 1 : 0;
     *ctrl = c;
+}
+
+__device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
+                                          Ctrl* __restrict__ ctrl, const VwLog* lg = nullptr) {
+    __shared__ double sb[kCtrlThreads];
+    __shared__ double stt[kCtrlThreads];
+    __shared__ long long sn[kCtrlThreads];
+    const int t = threadIdx.x;
+    // the partials are read before the halt flag is known (a halted step
+    // leaves them as they were), so these loads and the control block's
+    // (uniform: scalar loads) are in flight together; nothing in this
+    // kernel writes the control block before its last store
+    const Ctrl c0 = *ctrl;
+    double ab = 0.0, at = 0.0;
+    long long nb = 0;
+#pragma unroll 4
+    for (int b = t; b < p.nblk; b += kCtrlThreads) {
+        ab = ab + s.part_b[b];
+        at = at + s.part_t[b];
+        nb += s.part_bonds[b];
+    }
+    if (c0.halt) return;
+    // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
+    // one shuffle per level
+    sb[t] = ab;
+    stt[t] = at;
+    sn[t] = nb;
+    __syncthreads();
+    for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
+        if (t < off) {
+            sb[t] = sb[t] + sb[t + off];
+            stt[t] = stt[t] + stt[t + off];
+            sn[t] += sn[t + off];
+        }
+        __syncthreads();
+    }
+    if (t >= 64) return;
+    double vb = sb[t], vt = stt[t];
+    long long vn = sn[t];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        vb = vb + __shfl_down(vb, off);
+        vt = vt + __shfl_down(vt, off);
+        vn += __shfl_down(vn, off);
+    }
+    if (t != 0) return;
+    ctrl_logic(s, p, ctrl, c0, vb, vt, vn, lg);
 }
 
 __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevParams p,
