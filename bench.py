@@ -25,7 +25,8 @@ sys.path.insert(0, os.path.join(ROOT, "dem-rock-calibration_amd"))
 HBM_PEAK_GBPS = 8000.0
 # Step-kernel launches of the timed region are sampled with HIP events: each
 # 256-step chunk runs as a captured HIP graph of 255 steps plus one step
-# launched on its own between two events on the same stream.
+# launched on its own whose two events take the dispatch's own begin and end
+# timestamps (hipExtLaunchKernelGGL on the context's stream).
 TIMING_STRIDE = 8
 # Measured HBM traffic of one step-kernel launch (rocprofv3 FETCH_SIZE and
 # WRITE_SIZE, scaled by a copy kernel of known size), for the default workload.
@@ -196,7 +197,7 @@ def batch_bench(count, steps, warmup):
     }))
 
 
-def grad_bench(steps=2000, n=20000):
+def grad_bench(steps=2000, n=20000, checks=True):
     """Cost of the loading-response gradient against the forward run it
     differentiates: a 20k-sphere cylinder (r ~ U[0.20, 0.30] mm) loaded for
     2000 steps to 0.1 % strain (closing speed 0.001 H / (2000 dt)), three
@@ -234,6 +235,13 @@ def grad_bench(steps=2000, n=20000):
     fwd = min(timed(False)[0] for _ in range(3))
     grad = min(timed(True)[0] for _ in range(3))
     log(f"forward {fwd * 1e3:.1f} ms, gradient ({len(names)} directions) {grad * 1e3:.1f} ms")
+    if not checks:
+        print(json.dumps({
+            "metric": "gradient cost / forward pre-peak run (20k spheres, 2000 steps, 3 directions)",
+            "value": grad / fwd, "unit": "x forward", "n_gpus": 1, "steps": steps,
+            "higher_is_better": False, "dtype": "f64",
+            "config": {"forward_ms": fwd * 1e3, "gradient_ms": grad * 1e3, "directions": names}}))
+        return
     setup = dict(xyz=w["xyz"], r=w["r"], mat=mat, box=w["box"], cfg=cfg, steps=steps)
     s0, J, e0 = autograd.run_window(setup, names, theta, sens=True, strain=True)
     # points before the first platen pause (a controller event the tangents
@@ -344,6 +352,8 @@ def main():
                     help="measure a batch of this many samples against sequential runs and exit")
     ap.add_argument("--grad", action="store_true",
                     help="measure the gradient run against the forward run and exit")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="with --grad: the cost only, without the finite-difference checks")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -352,7 +362,7 @@ def main():
         batch_bench(args.batch, args.steps, args.warmup)
         return
     if args.grad:
-        grad_bench()
+        grad_bench(checks=not args.no_fd)
         return
     if args.copy_probe > 0:
         import torch

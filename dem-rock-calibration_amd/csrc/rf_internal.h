@@ -200,6 +200,10 @@ hipError_t launch_set_stop(Ctrl* ctrl, int64_t stop_at, hipStream_t st);
 hipError_t launch_set_strengths(BondRec* brec, const double* sc, const double* tc, int64_t n,
                                 hipStream_t st);
 hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
+// The same launch with `start` and `stop` taking the dispatch's own begin and
+// end timestamps (hipExtLaunchKernelGGL): the kernel's execution time only.
+hipError_t launch_step_timed(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st,
+                             hipEvent_t start, hipEvent_t stop);
 hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st);
 hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample,
                              const int32_t* blk_local, int32_t nblk_total,

@@ -31,6 +31,8 @@
 // Floating point: compiled with -ffp-contract=off; sqrt and division are the
 // IEEE-rounded FP64 operations; expressions are evaluated left to right as
 // written, in the same order as the CPU restatement.
+#include <hip/hip_ext.h>
+
 #include "rf_internal.h"
 #include "rf_physics.h"
 
@@ -1366,6 +1368,13 @@ hipError_t launch_clear_flags(Ctrl* ctrl, hipStream_t st) {
 
 hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st) {
     hipLaunchKernelGGL(k_step, dim3((unsigned)p.nblk), dim3(kBlock), 0, st, s, p, ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_timed(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st,
+                             hipEvent_t start, hipEvent_t stop) {
+    hipExtLaunchKernelGGL(k_step, dim3((unsigned)p.nblk), dim3(kBlock), 0, st, start, stop, 0, s, p,
+                          ctrl);
     return hipGetLastError();
 }
 

@@ -802,23 +802,24 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
         }
         const int64_t before = c->hctrl.step;
         // A full chunk runs as a captured graph; with timing on, its last step
-        // is launched on its own between two events (one sample per chunk).
+        // is launched on its own with two events that take the dispatch's own
+        // timestamps (one sample per chunk).
         const bool graphed = c->use_graph && k == c->chunk_eff && k > 1;
         if (graphed) {
             st = graph_chunk(c, c->timing ? k - 1 : k);
             if (st) return st;
             if (c->timing) {
-                HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-                HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
-                HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+                HIP_TRY(rf::launch_step_timed(c->d, c->p, c->ctrl, c->stream, c->ev[0], c->ev[1]));
                 HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
             }
         } else {
             for (int64_t s = 0; s < k; ++s) {
                 const bool tm = c->timing && (before + s) % c->timing == 0;
-                if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s], c->stream));
-                HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
-                if (tm) HIP_TRY(hipEventRecord(c->ev[2 * s + 1], c->stream));
+                if (tm)
+                    HIP_TRY(rf::launch_step_timed(c->d, c->p, c->ctrl, c->stream, c->ev[2 * s],
+                                                  c->ev[2 * s + 1]));
+                else
+                    HIP_TRY(rf::launch_step(c->d, c->p, c->ctrl, c->stream));
                 HIP_TRY(rf::launch_controller(c->d, c->p, c->ctrl, c->stream));
             }
         }

@@ -299,10 +299,12 @@ rf_status rf_set_state(rf_ctx* ctx, const rf_state_view* in);
 int64_t rf_num_particles(const rf_ctx* ctx);
 int32_t rf_num_slots(const rf_ctx* ctx);
 
-/* Timing: rf_set_timing(ctx, k) with k > 0 brackets every k-th launch of
+/* Timing: rf_set_timing(ctx, k) with k > 0 times every k-th launch of
  * the fused step kernel (global step index divisible by k; in a chunk run as
- * a graph, its last step instead) with HIP events on the context's stream;
- * 0 switches it off.  rf_last_timing returns, for
+ * a graph, its last step instead) with two HIP events that take the
+ * dispatch's own begin and end timestamps (hipExtLaunchKernelGGL), so a
+ * sample is the kernel's execution time without the launch gap; 0 switches
+ * it off.  rf_last_timing returns, for
  * the last rf_step call, the summed time of the bracketed launches (ms), how
  * many launches were bracketed, the wall time of the call (ms) and the
  * number of steps it executed.  No counterpart in the reference. */
@@ -323,7 +325,7 @@ rf_status rf_capture_chunk(rf_ctx* ctx, int32_t steps);
 
 /* Replay each full chunk of rf_step as one captured HIP graph (default on).
  * With timing on, the last step of each graphed chunk is launched on its own
- * between events (one timing sample per chunk).  Results are the same either
+ * with timing events (one timing sample per chunk).  Results are the same either
  * way.  No counterpart in the reference. */
 rf_status rf_set_graph(rf_ctx* ctx, int32_t enable);
 
