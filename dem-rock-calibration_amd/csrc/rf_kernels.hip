@@ -190,23 +190,20 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     __shared__ double sh_b[kPerBlock];
     __shared__ double sh_t[kPerBlock];
     __shared__ int sh_nb[kPerBlock];
-    if (ctrl->halt) return;  // uniform for the whole grid
     const int l = threadIdx.x & (kLanes - 1);
     const int grp = threadIdx.x / kLanes;
     const int64_t i = (int64_t)blk * kPerBlock + grp;
     const bool valid = i < p.n;
+    // The control block (written by the previous controller launch) and the
+    // particle's own rows, which do not depend on it, are requested together;
+    // the halt test (uniform for the whole grid) waits for both.
+    const int32_t halt = ctrl->halt;
     const int cur = ctrl->cur;
-    const double4* __restrict__ pos = s.pos[cur];
-    const double4* __restrict__ vel = s.vel[cur];
-    const double4* __restrict__ angv = s.angv[cur];
     const double top = ctrl->top, bottom = ctrl->bottom, vw = ctrl->vw;
     const int S = p.S, C = p.C, B = p.B;
     const int64_t ic = valid ? i : 0;  // clamped index for the inactive groups' loads
 
     // ---- stage 1: per-particle loads
-    const double4 pi = pos[ic];
-    const double4 vi = vel[ic];
-    const double4 wi = angv[ic];
     const double2 pk = s.pk[ic];
     const double4 rb = s.rbuild[ic];
     const uint32_t occ_raw = s.occ[ic], bnd_raw = s.bnd[ic], otnz_raw = s.otnz[ic];
@@ -264,6 +261,19 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         }
     }
 #endif
+
+    // both buffers' addresses in registers before the test, so the select
+    // below is not turned back into a kernel-argument load after it
+    using GPos = const __attribute__((address_space(1))) double4*;  // global memory
+    GPos pos0 = (GPos)s.pos[0], pos1 = (GPos)s.pos[1];
+    asm volatile("" : "+s"(pos0), "+s"(pos1));
+    if (halt) return;
+    const double4* __restrict__ pos = (const double4*)(cur ? pos1 : pos0);
+    const double4* __restrict__ vel = cur ? s.vel[1] : s.vel[0];
+    const double4* __restrict__ angv = cur ? s.angv[1] : s.angv[0];
+    const double4 pi = pos[ic];
+    const double4 vi = vel[ic];
+    const double4 wi = angv[ic];
 
     // ---- stage 2: gathers (clamped to particle i where not needed)
     double4 pp[kPerLane];
