@@ -91,6 +91,11 @@ struct rf_ctx {
     rf::DevParams p{};
     rf::Ctrl* ctrl = nullptr;
     rf::Ctrl hctrl{};
+    // pinned staging of the control block's copies to the host (a copy into
+    // pageable memory goes through a driver staging buffer), and the two
+    // events that bracket an rf_step call; made on first use
+    rf::Ctrl* hpin = nullptr;
+    hipEvent_t wall_a = nullptr, wall_b = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     bool built = false;
@@ -458,6 +463,9 @@ rf_status rf_destroy(rf_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev) hipEventDestroy(e);
+    if (c->wall_a) hipEventDestroy(c->wall_a);
+    if (c->wall_b) hipEventDestroy(c->wall_b);
+    if (c->hpin) hipHostFree(c->hpin);
     for (auto& g : c->graphs)
         if (g.exec) hipGraphExecDestroy(g.exec);
     for (void* ptr : c->allocs) hipFree(ptr);
@@ -473,9 +481,17 @@ int32_t rf_num_slots(const rf_ctx* c) { return c ? c->S : 0; }
 
 namespace {
 
+rf_status ensure_pinned(rf_ctx* c) {
+    if (!c->hpin) HIP_TRY(hipHostMalloc((void**)&c->hpin, sizeof(rf::Ctrl), hipHostMallocDefault));
+    return RF_OK;
+}
+
 rf_status sync_ctrl(rf_ctx* c) {
-    HIP_TRY(hipMemcpyAsync(&c->hctrl, c->ctrl, sizeof(rf::Ctrl), hipMemcpyDeviceToHost, c->stream));
+    rf_status st = ensure_pinned(c);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(c->hpin, c->ctrl, sizeof(rf::Ctrl), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->hctrl = *c->hpin;
     return RF_OK;
 }
 
@@ -490,8 +506,10 @@ rf_status device_error(rf_ctx* c) {
     return fail(RF_ERR_STATE, "device error flags 0x%x", e);
 }
 
-// Ends a capture begun on `st` whatever happened inside it and instantiates
-// the graph; `launch_err` is the first error a launch inside it returned.
+// Ends a capture begun on `st` whatever happened inside it, instantiates the
+// graph and uploads it to the device (otherwise its first launch uploads it:
+// about 100 us for a 40-node graph, all of it inside the first rf_step that
+// replays it); `launch_err` is the first error a launch inside it returned.
 rf_status end_capture(hipStream_t st, hipError_t launch_err, hipGraphExec_t* exec) {
     *exec = nullptr;
     hipGraph_t gr = nullptr;
@@ -499,6 +517,10 @@ rf_status end_capture(hipStream_t st, hipError_t launch_err, hipGraphExec_t* exe
     hipError_t e = launch_err != hipSuccess ? launch_err : ee;
     if (e == hipSuccess) e = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
     if (gr) hipGraphDestroy(gr);
+    if (e == hipSuccess) {
+        e = hipGraphUpload(*exec, st);
+        if (e != hipSuccess) hipGraphExecDestroy(*exec);
+    }
     if (e != hipSuccess) {
         *exec = nullptr;
         return fail(RF_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -779,16 +801,9 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
     if (st) return st;
     const int64_t start = c->hctrl.step;
     int64_t remaining = nsteps;
-    struct EventPair {
-        hipEvent_t a = nullptr, b = nullptr;
-        ~EventPair() {
-            if (a) hipEventDestroy(a);
-            if (b) hipEventDestroy(b);
-        }
-    } wall;
-    HIP_TRY(hipEventCreate(&wall.a));
-    HIP_TRY(hipEventCreate(&wall.b));
-    HIP_TRY(hipEventRecord(wall.a, c->stream));
+    if (!c->wall_a) HIP_TRY(hipEventCreate(&c->wall_a));
+    if (!c->wall_b) HIP_TRY(hipEventCreate(&c->wall_b));
+    HIP_TRY(hipEventRecord(c->wall_a, c->stream));
     double kernel_ms = 0.0;
     int64_t timed = 0;
     while (remaining > 0 && !c->hctrl.done) {
@@ -854,10 +869,10 @@ rf_status rf_step(rf_ctx* c, int64_t nsteps, int64_t* done_steps) {
             return fail(RF_ERR_STATE, "loading loop halted without a reason");
         }
     }
-    HIP_TRY(hipEventRecord(wall.b, c->stream));
-    HIP_TRY(hipEventSynchronize(wall.b));
+    HIP_TRY(hipEventRecord(c->wall_b, c->stream));
+    HIP_TRY(hipEventSynchronize(c->wall_b));
     float total = 0.f;
-    HIP_TRY(hipEventElapsedTime(&total, wall.a, wall.b));
+    HIP_TRY(hipEventElapsedTime(&total, c->wall_a, c->wall_b));
     c->last_total_ms = total;
     c->last_kernel_ms = kernel_ms;
     c->last_timed = timed;
@@ -1412,10 +1427,14 @@ rf_status batch_launch(rf_batch* b, int64_t k, bool graphed) {
 }
 
 rf_status batch_sync(rf_batch* b) {
-    for (rf_ctx* c : b->c)
-        HIP_TRY(hipMemcpyAsync(&c->hctrl, c->ctrl, sizeof(rf::Ctrl), hipMemcpyDeviceToHost,
+    for (rf_ctx* c : b->c) {
+        rf_status st = ensure_pinned(c);
+        if (st) return st;
+        HIP_TRY(hipMemcpyAsync(c->hpin, c->ctrl, sizeof(rf::Ctrl), hipMemcpyDeviceToHost,
                                b->stream));
+    }
     HIP_TRY(hipStreamSynchronize(b->stream));
+    for (rf_ctx* c : b->c) c->hctrl = *c->hpin;
     return RF_OK;
 }
 
