@@ -371,8 +371,18 @@ def main():
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
+        # measured copy bandwidth (bytes read + written per second), the
+        # attainable counterpart of the 8 TB/s peak
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3}))
+        ms = e0.elapsed_time(e1) / reps
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "copies": 3 + reps,
+                          "copy_ms": ms, "copy_GBps": 2 * args.copy_probe / (ms * 1e-3) / 1e9}))
         return
     if args.gpus != 1:
         print(f"bench.py: the DEM step runs on one GPU; --gpus {args.gpus} is not supported "

@@ -57,35 +57,6 @@ def target_functional(points, target, weights):
     return out
 
 
-def normalized_parameters(p, coeffs, n):
-    """NormalizedParameters (NN_MP.py:408-421)."""
-    return [p[i] / coeffs[i] for i in range(n)]
-
-
-def renormalized_parameters(p, coeffs, n):
-    """ReNormalizedParameters (NN_MP.py:424-429)."""
-    return [p[i] * coeffs[i] for i in range(n)]
-
-
-def local_points(X, Y, variation, external_variation, n_min, n_param):
-    """GetNewArrayOfPoints (NN_MP.py:506-543): the best point, the points
-    inside its external window [e/v_ext, e*v_ext] on every parameter, and
-    how many random runs are missing to reach n_min.  Returns
-    (n_random, X_local, Y_local, border, index_best)."""
-    best = int(np.argmin(Y))
-    m = X[best]
-    border = [(e / variation, e * variation) for e in m]
-    outer = [(e / external_variation, e * external_variation) for e in m]
-    Xl, Yl = [], []
-    for i in range(len(X)):
-        e = X[i]
-        if all(outer[j][0] <= e[j] <= outer[j][1] for j in range(n_param)):
-            Xl.append(X[i])
-            Yl.append(Y[i])
-    n_rand = max(n_min - len(Yl), 0)
-    return n_rand, Xl, Yl, border, best
-
-
 def cg_functional(results, goal, weight_mult):
     """The CG's weighted squares W_j (G_j - R_j)^2 with W_j = mult_j / G_j^2
     (CalculateStartParameters and CalculateNewParameters2, mp_task.h:138-139,

@@ -1,5 +1,5 @@
 """Host parts of the calibration drivers (rockfit.calibrate): the farm's
-objective and local window (NN_MP.py:408-543) and the CG functional
+objective (NN_MP.py:461-504, 849) and the CG functional
 (mp_task.h:138-139, 275-279), on hand-computed cases.  The reference holds
 no fixtures for them; each case below is worked out from the rules the
 docstrings state."""
@@ -37,27 +37,6 @@ def test_objective_penalties_and_cap():
 def test_farm_features():
     row = np.arange(14, dtype=float)
     np.testing.assert_array_equal(calibrate.farm_features(row), [6, 7, 8, 9, 11, 13])
-
-
-def test_normalisation_round_trip():
-    c = [2e11, 1e9, 1e9, 5e-4]
-    p = [1.7e10, 6e8, 5e8, 4e-4]
-    n = calibrate.normalized_parameters(p, c, 4)
-    assert n == [p[i] / c[i] for i in range(4)]
-    assert calibrate.renormalized_parameters(n, c, 4) == pytest.approx(p, rel=1e-15)
-
-
-def test_local_points():
-    X = [[1.0, 1.0], [1.2, 0.9], [3.0, 1.0], [1.0, 1.6], [0.8, 1.1]]
-    Y = [5.0, 1.0, 0.5, 2.0, 3.0]
-    # best is X[2]; window [e/1.3, 1.3e] holds only itself
-    n_rand, Xl, Yl, border, best = calibrate.local_points(X, Y, 1.1, 1.3, 4, 2)
-    assert best == 2 and Xl == [[3.0, 1.0]] and Yl == [0.5] and n_rand == 3
-    assert border[0] == pytest.approx((3.0 / 1.1, 3.3), rel=1e-15)
-    Y = [5.0, 1.0, 7.0, 2.0, 3.0]
-    n_rand, Xl, Yl, _, best = calibrate.local_points(X, Y, 1.1, 1.5, 2, 2)
-    # best X[1] = (1.2, 0.9): window x in [0.8, 1.8], y in [0.6, 1.35]
-    assert best == 1 and Xl == [[1.0, 1.0], [1.2, 0.9], [0.8, 1.1]] and n_rand == 0
 
 
 def test_cg_functional():
