@@ -968,14 +968,22 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller_batch(const BatchEn
 // ------------------------------------------------------ cell binning ------
 // d_CalculateCellIndex (md_calculate_cell.cu:13-41) plus a grid check the
 // reference only prints under pre_debugtest.
+// Whether cell (fx, fy, fz) lies in 1..dims-2 along every axis, so that its
+// whole 27-cell neighbourhood is in the grid.  A particle outside is flagged
+// (kErrGrid) by k_cell_keys, and the kernels that scan a stencil skip it;
+// the host then reports RF_ERR_GRID.
+__device__ __forceinline__ bool in_grid(const DevParams& p, double fx, double fy, double fz) {
+    return fx >= 1.0 && fx <= p.nx - 2 && fy >= 1.0 && fy <= p.ny - 2 && fz >= 1.0 &&
+           fz <= p.nz - 2;
+}
+
 __global__ void k_cell_keys(DevState s, DevParams p, Ctrl* ctrl) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.n) return;
     const double4 r = s.pos[ctrl->cur][i];
     const double fx = floor(r.x * p.inv_cell), fy = floor(r.y * p.inv_cell),
                  fz = floor(r.z * p.inv_cell);
-    if (!(fx >= 1.0 && fx <= p.nx - 2 && fy >= 1.0 && fy <= p.ny - 2 && fz >= 1.0 &&
-          fz <= p.nz - 2)) {
+    if (!in_grid(p, fx, fy, fz)) {
         atomicOr(&ctrl->err, kErrGrid);
         s.ckey[i] = 0;
         return;
@@ -1049,6 +1057,12 @@ __global__ void k_construct_rows(DevState s, DevParams p, Ctrl* ctrl) {
     const double4 pi = pos[i];
     const double fx = floor(pi.x * p.inv_cell), fy = floor(pi.y * p.inv_cell),
                  fz = floor(pi.z * p.inv_cell);
+    if (!in_grid(p, fx, fy, fz)) {  // flagged by k_cell_keys: an empty row
+        s.occ[i] = 0u;
+        s.bnd[i] = 0u;
+        s.otnz[i] = 0u;
+        return;
+    }
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t base = (int64_t)fx + (int64_t)fy * nx + (int64_t)fz * nxy;
     const int64_t sx = copysign(1.0, pi.x - (fx + 0.5) * p.cell) > 0.0 ? 1 : -1;
@@ -1157,7 +1171,11 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     const int64_t ic = valid ? i : 0;
     const double4* pos = s.pos[ctrl->cur];
     const double4 pi = pos[ic];
-    const bool skip = !valid | ((pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]));
+    // particles at the hidden point, and particles outside the grid (flagged
+    // by k_cell_keys), get no candidates
+    const bool skip = !valid | ((pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2])) |
+                      !in_grid(p, floor(pi.x * p.inv_cell), floor(pi.y * p.inv_cell),
+                               floor(pi.z * p.inv_cell));
     const int64_t key = s.ckey[ic];
     const int64_t nx = p.nx, nxy = (int64_t)p.nx * p.ny;
     const int64_t C = p.C;
