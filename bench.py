@@ -118,7 +118,7 @@ def cpu_baseline(w, threads, seconds=12.0):
                                "sample": f"{k_one} loading steps, 1 thread"})
 
 
-def batch_bench(count, steps, warmup):
+def batch_bench(count, steps, warmup, cache=None):
     """Many samples at once (rf_batch_*) against the same samples run one after
     another on the GPU (the reference's TestSeveralSamples pattern,
     mp_testseveralsamples.cu:119-161).  Samples: N ~ log-uniform [2k, 40k],
@@ -131,7 +131,7 @@ def batch_bench(count, steps, warmup):
     from rockfit.core import Batch, Context
 
     t0 = time.perf_counter()
-    ws = config.batch_workloads(count)
+    ws = config.batch_workloads(count, cache_dir=cache)
     total_n = int(sum(w["n"] for w in ws))
     log(f"{count} samples, {total_n} particles, generated in {time.perf_counter() - t0:.1f}s")
 
@@ -180,7 +180,8 @@ def batch_bench(count, steps, warmup):
         "warmup": warmup, "higher_is_better": True, "dtype": "f64",
         "data": "synthetic (seeded random sequential insertion + FIRE on the GPU)",
         "config": {"workload": "batch of uniaxial tests", "samples": count,
-                   "particles": total_n},
+                   "particles": total_n,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "sequential": {"value": pst_seq, "unit": "particle-steps/s",
                        "kind": "same samples one after another on the GPU"},
         "speedup": pst_bat / pst_seq,
@@ -352,14 +353,22 @@ def main():
                     help="measure a batch of this many samples against sequential runs and exit")
     ap.add_argument("--grad", action="store_true",
                     help="measure the gradient run against the forward run and exit")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="hardware queues of the process (GPU_MAX_HW_QUEUES, HIP's default 4, "
+                         "at most 32): how many samples' kernels the batch's concurrent form can "
+                         "run side by side")
     ap.add_argument("--no-fd", action="store_true",
                     help="with --grad: the cost only, without the finite-difference checks")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.hw_queues:
+        if not 1 <= args.hw_queues <= 32:
+            sys.exit("bench.py: --hw-queues must be in 1..32")
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)  # read when HIP starts
     if args.batch > 0:
-        batch_bench(args.batch, args.steps, args.warmup)
+        batch_bench(args.batch, args.steps, args.warmup, cache=args.cache)
         return
     if args.grad:
         grad_bench(checks=not args.no_fd)

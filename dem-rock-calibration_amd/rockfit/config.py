@@ -234,11 +234,15 @@ def config_workload(name, n=None, seed=None, dnsave=100, cache=None, **test_kwar
     return w
 
 
-def batch_workloads(count=64, seed=SEED + 4, n_lo=2000, n_hi=40000):
+def batch_workloads(count=64, seed=SEED + 4, n_lo=2000, n_hi=40000, cache_dir=None):
     """C4: `count` uniaxial samples, N log-uniform in [n_lo, n_hi], radii
     U[0.20, 0.30] mm, micro-parameters on a Latin hypercube: bond E in
     [5, 50] GPa, normal strength in [10, 100] MPa (shear strength 1.5 times
-    it), friction in [0.3, 0.8]; loaded at strain rate 0.1/s."""
+    it), friction in [0.3, 0.8]; loaded at strain rate 0.1/s.  cache_dir:
+    a directory holding each sample's generator output (make_workload's
+    cache), so later runs skip the generation."""
+    if cache_dir is not None:
+        os.makedirs(cache_dir, exist_ok=True)
     rng = np.random.default_rng(seed)
     ns = np.exp(rng.uniform(np.log(n_lo), np.log(n_hi), count)).astype(int)
 
@@ -251,6 +255,8 @@ def batch_workloads(count=64, seed=SEED + 4, n_lo=2000, n_hi=40000):
     for k in range(count):
         w = make_workload(n=int(ns[k]), r_min=0.20, r_max=0.30, test="uniaxial",
                           seed=seed + 100 + k, generator="rsi",
+                          cache=(None if cache_dir is None else
+                                 os.path.join(cache_dir, f"batch_{seed}_{k}_{int(ns[k])}.npz")),
                           mat_kwargs=dict(E_bond_gpa=float(E[k]), sigma_mpa=float(sig[k]),
                                           tau_mpa=float(1.5 * sig[k]), friction=float(mu[k])))
         v = -STRAIN_RATE * w["cfg"].size[2] * 1e-3
