@@ -196,3 +196,44 @@ def test_derived_dt_refuses_the_parameters_that_move_it():
         assert e.value.status == abi.RF_ERR_STATE
     c.sens_begin([autograd.PARAMS["friction"][0], autograd.PARAMS["rolling_friction"][0]])
     assert c.sens_step(5) == 5
+
+
+def test_bond_stretched_past_acut_ends_the_window():
+    """A bond stretched past acut, which the plain step drops without a break
+    (its prune), ends a sensitivity window with RF_ERR_STATE instead of being
+    evaluated on (rf_sens.hip's slot pass)."""
+    from rockfit import abi
+    from rockfit._lib import RockfitError
+
+    w = config.make_workload(n=600, r_min=0.2, r_max=0.3, test="uniaxial", seed=3)
+    mat = autograd.fixed_dt(w["mat"], w["r"])
+    # one bonded pair of the sample, on its own
+    full = Context(w["xyz"], w["r"], mat, w["box"])
+    full.build_bonds()
+    st = full.get_state()
+    S = full.slots
+    q = int(np.nonzero(st["type"] == 1)[0][0])
+    pair = [q // S, int(st["nbr"][q])]
+    xyz, r = w["xyz"][pair], w["r"][pair]
+    xyz, box = config.place_in_box(xyz, r, mat)
+    print("case bond stretched past acut", flush=True)
+
+    def stretched():
+        c = Context(xyz, r, mat, box)
+        c.build_bonds()
+        s = c.get_state()
+        assert np.count_nonzero(s["type"] == 1) == 2  # one bond, both directions
+        R = s["R"].reshape(3, 2)
+        d = R[:, 1] - R[:, 0]
+        R[:, 1] = R[:, 0] + d / np.linalg.norm(d) * 1.2 * mat.acut
+        c.set_state(s)
+        return c
+
+    plain = stretched()
+    assert plain.step(1) == 1
+    assert np.count_nonzero(plain.get_state()["type"] == 1) == 0  # pruned, not broken
+    sens = stretched()
+    sens.sens_begin([autograd.PARAMS["friction"][0]])
+    with pytest.raises(RockfitError) as e:
+        sens.sens_step(1)
+    assert e.value.status == abi.RF_ERR_STATE
