@@ -1,13 +1,18 @@
-"""Mutation check of the history pins (tests/test_oracle_history.py).
+"""Mutation check of the history pins (tests/test_oracle_history.py) and of
+the lattice macro pin (tests/test_oracle_macro.py).
 
 Each mutant is a copy of oracle/dem_oracle.c with one plausible slip in the
 history transport of md_calculateforcesdem3.cu:57-101, 113-118, 138-145,
 183-186, 264-279: the sign of one term of one of the nine entries of M
 flipped (24 mutants), M transposed, t or the factor 1/2 of phi wrong, the
 previous normal not stored, or, for contacts, the projection or the rescale
-dropped or the rescale with a wrong index.  Every mutant is compiled with gcc
-and must fail the history module; the unmutated source, compiled the same
-way, must pass it.  CPU only.
+dropped or the rescale with a wrong index.  Four more carry a slip in the
+macro response: the stress area without r_ref (md_data_types.h:212-242), a
+platen moving a full instead of a half step (:347-379), a doubled bond
+cross-section (md_calculateforcesdem3.cu:20-200), a halved wall modulus
+(md_calculationuniaxialcompression.cu:278).  Every mutant is compiled with
+gcc and must fail its module; the unmutated source, compiled the same way,
+must pass both.  CPU only.
 """
 import os
 import subprocess
@@ -66,6 +71,18 @@ MUTANTS += [
 ]
 
 
+# Slips in the macro response, run against tests/test_oracle_macro.py.
+MACRO_MUTANTS = [
+    ("area_without_r_ref", "int ora_set_test(",
+     "c->area = PI * (hx + cfg->r_ref) * (hx + cfg->r_ref);", "c->area = PI * hx * hx;"),
+    ("platen_full_step", "static void controller(", "c->top += 0.5 * c->vw * dt;",
+     "c->top += c->vw * dt;"),
+    ("bond_section_doubled", BOND, "cefn *= b * b * PI;", "cefn *= 2.0 * b * b * PI;"),
+    ("wall_modulus_halved", "double Ew = c->cfg.wall_modulus_factor",
+     "c->cfg.wall_modulus_factor * c->der.hm_E", "c->der.hm_E"),
+]
+
+
 def _mutate(src, anchor, old, new):
     a = src.index(anchor)
     q = src.index(old, a)
@@ -84,10 +101,10 @@ def _build(tmp, name, text):
     return so
 
 
-def _pins_pass(so):
+def _pins_pass(so, module="test_oracle_history.py"):
     env = dict(os.environ, RF_ORACLE_LIB=so)
     r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_history.py")],
+                        os.path.join(ROOT, "tests", module)],
                        cwd=ROOT, env=env, capture_output=True, text=True)
     return r.returncode == 0
 
@@ -112,7 +129,26 @@ def test_every_mutant_fails_the_history_pins(tmp_path):
     assert not survivors, f"mutants not caught by the history pins: {survivors}"
 
 
-@pytest.mark.parametrize("name,anchor,old,new", MUTANTS[:1])
+def test_every_macro_mutant_fails_the_lattice_pin(tmp_path):
+    src = open(SRC).read()
+    jobs = [("control", src)]
+    for name, anchor, old, new in MACRO_MUTANTS:
+        text = _mutate(src, anchor, old, new)
+        assert text != src, name
+        jobs.append((name, text))
+
+    def run(job):
+        name, text = job
+        return name, _pins_pass(_build(str(tmp_path), name, text), "test_oracle_macro.py")
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        results = dict(ex.map(run, jobs))
+    assert results.pop("control"), "the unmutated oracle must pass the lattice pin"
+    survivors = [name for name, passed in results.items() if passed]
+    assert not survivors, f"mutants not caught by the lattice pin: {survivors}"
+
+
+@pytest.mark.parametrize("name,anchor,old,new", MUTANTS[:1] + MACRO_MUTANTS)
 def test_mutation_applies(name, anchor, old, new):
     src = open(SRC).read()
     assert old in src[src.index(anchor):]
