@@ -10,12 +10,17 @@
 
 namespace rf {
 
-// Particles handled by one 256-thread block of the step kernel: one 8-lane
-// group per particle, 32 groups per block.  Lane l of a group owns slots,
-// bonds and candidates l, l+8, l+16, l+24.
+// Step kernel: one 8-lane group per particle, lane l of a group owning
+// slots, bonds and candidates l, l+8, l+16, l+24; one wave (8 particles) per
+// workgroup, so a wave's slot is refilled as soon as it ends.  The platen
+// forces are summed per group of 32 consecutive particles (a tree); wave w
+// of group b holds particles 32 b + w + 4 k, k = 0..7, which puts the
+// tree's first three levels inside the wave (see step_body).
 constexpr int kLanes = 8;
-constexpr int kBlock = 256;
-constexpr int kPerBlock = kBlock / kLanes;  // 32 particles per block
+constexpr int kBlock = 64;                  // threads of a step workgroup: one wave
+constexpr int kPerWave = kBlock / kLanes;   // 8 particles per wave
+constexpr int kGroup = 32;                  // particles per platen-partial group
+constexpr int kWavesPerGroup = kGroup / kPerWave;  // 4
 constexpr int kPerLane = 4;                 // entries per lane and list
 constexpr int kMaxSlots = kLanes * kPerLane;  // 32 slots per particle
 constexpr int kMaxCand = kLanes * kPerLane;   // 32 candidates per particle
@@ -78,7 +83,11 @@ struct DevState {
     // order; ids rank new pairs and order the cells, as in the reference)
     int32_t* orig;
     // reductions
+    // per group of 32 particles, one entry per wave: [4 b + w]
     double* part_b; double* part_t; int32_t* part_bonds;
+    // bit b: group b has a non-zero platen partial this step (set by the
+    // step kernel, read and cleared by the controller)
+    uint32_t* pmask;
     // curve
     double* curve; int64_t* bcount;
 };

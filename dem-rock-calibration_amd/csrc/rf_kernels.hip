@@ -185,14 +185,14 @@ __device__ __forceinline__ void contact_eval(const DevParams& p, const double4& 
 // its result masked afterwards, so a wave waits on two memory round trips
 // before any force work.  Contacts gather in rounds of one slot per lane.
 // Bonds beyond the 8th and appends are rare paths.
+// `wave`: wave w = wave % 4 of particle group b = wave / 4, holding
+// particles 32 b + w + 4 k (k = 0..7, the wave's 8-lane groups).
 __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
-                                          Ctrl* __restrict__ ctrl, const int blk) {
-    __shared__ double sh_b[kPerBlock];
-    __shared__ double sh_t[kPerBlock];
-    __shared__ int sh_nb[kPerBlock];
+                                          Ctrl* __restrict__ ctrl, const int wave) {
     const int l = threadIdx.x & (kLanes - 1);
     const int grp = threadIdx.x / kLanes;
-    const int64_t i = (int64_t)blk * kPerBlock + grp;
+    const int64_t gbase = (int64_t)(wave / kWavesPerGroup) * kGroup + wave % kWavesPerGroup;
+    const int64_t i = gbase + (int64_t)kWavesPerGroup * grp;
     const bool valid = i < p.n;
     // The control block (written by the previous controller launch) and the
     // particle's own rows, which do not depend on it, are requested together;
@@ -329,7 +329,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const int32_t bjs = __shfl(pre1 ? bp1 : (int32_t)ic, from);
         const int bss = __shfl(pre1 ? bs1 : -1, from);
         if (from != lane) {
-            const int64_t io = (int64_t)blk * kPerBlock + (int64_t)((threadIdx.x & ~63u) + from) / kLanes;
+            const int64_t io = gbase + (int64_t)kWavesPerGroup * (from / kLanes);
             bjx = bjs;
             qbx = bss >= 0 ? io * B + kLanes + (from & (kLanes - 1)) : io * B;
             bsx = bss;
@@ -638,6 +638,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     }
 
     // ---- 3. totals, platens, kick-drift, cut (lane 0 of the group)
+    double gb = -0.0, gt = -0.0;  // the particle's platen forces (lane 0; -0.0 past the last)
+    int gn = 0;                   // and its bonds
     if (valid && l == 0) {
         const double mi = vi.w;
         double part_b = -0.0, part_t = -0.0;
@@ -717,13 +719,9 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const bool was_hidden = (pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]);
         const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
         if ((out & !was_hidden) || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
-        sh_b[grp] = part_b;
-        sh_t[grp] = part_t;
-        sh_nb[grp] = __popc(bnd);
-    } else if (l == 0) {
-        sh_b[grp] = -0.0;
-        sh_t[grp] = -0.0;
-        sh_nb[grp] = 0;
+        gb = part_b;
+        gt = part_t;
+        gn = __popc(bnd);
     }
     // candidates whose slot was pruned are absent again
 #pragma unroll
@@ -732,25 +730,23 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         if (c < nc && c < C && cs[t] >= 0 && ((prune >> cs[t]) & 1u)) s.cslot[ic * C + c] = -1;
     }
 
-    // platen partials over the block's 32 particles, tree v[t] += v[t+off]
-    // (lane t of the first wave holds v[t]; every level is one shuffle)
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int t = (int)threadIdx.x;
-        double vb = t < kPerBlock ? sh_b[t] : 0.0;
-        double vt = t < kPerBlock ? sh_t[t] : 0.0;
-        int nb = t < kPerBlock ? sh_nb[t] : 0;
+    // Platen partials of the group's 32 particles, tree v[g] = v[g] + v[g+off],
+    // off = 16, 8, 4, 2, 1 (g = particle - 32 b).  The wave holds g = w + 4 k,
+    // so off = 16, 8, 4 pair its own groups k and k + 4, 2, 1 (lanes 8 k and
+    // 8 k + 32, 16, 8); lane 0 then holds v[w], and the controller adds the
+    // last two levels, (v[0] + v[2]) + (v[1] + v[3]).
 #pragma unroll
-        for (int off = kPerBlock / 2; off >= 1; off >>= 1) {
-            vb = vb + __shfl_down(vb, off);
-            vt = vt + __shfl_down(vt, off);
-            nb += __shfl_down(nb, off);
-        }
-        if (t == 0) {
-            s.part_b[blk] = vb;
-            s.part_t[blk] = vt;
-            s.part_bonds[blk] = nb;
-        }
+    for (int off = 32; off >= kLanes; off >>= 1) {
+        gb = gb + __shfl_down(gb, off);
+        gt = gt + __shfl_down(gt, off);
+        gn += __shfl_down(gn, off);
+    }
+    if (threadIdx.x == 0) {
+        s.part_b[wave] = gb;
+        s.part_t[wave] = gt;
+        s.part_bonds[wave] = gn;
+        const int g = wave / kWavesPerGroup;
+        if (gb != 0.0 || gt != 0.0) atomicOr(&s.pmask[g >> 5], 1u << (g & 31));
     }
 }
 
@@ -759,16 +755,17 @@ __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevPar
     step_body(s, p, ctrl, xcd_block((int)blockIdx.x, (int)gridDim.x));
 }
 
-// Many samples in one launch: logical block L runs block blk_local[L] of
-// sample blk_sample[L], so every sample runs exactly as it would alone; the
-// host chooses the order of the table (the batch autotuner), which never
-// changes a result.
+// Many samples in one launch: logical particle group G = L / 4 runs group
+// blk_local[G] of sample blk_sample[G] (its four waves are L = 4 G .. 4 G + 3),
+// so every sample runs exactly as it would alone; the host chooses the order
+// of the table (the batch autotuner), which never changes a result.
 __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step_batch(
     const BatchEntry* __restrict__ e, const int32_t* __restrict__ blk_sample,
     const int32_t* __restrict__ blk_local) {
     const int L = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const BatchEntry& b = e[blk_sample[L]];
-    step_body(b.s, b.p, b.ctrl, blk_local[L]);
+    const int G = L / kWavesPerGroup;
+    const BatchEntry& b = e[blk_sample[G]];
+    step_body(b.s, b.p, b.ctrl, blk_local[G] * kWavesPerGroup + L % kWavesPerGroup);
 }
 
 // -------------------------------------------------------- controller ------
@@ -866,12 +863,31 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     const Ctrl c0 = *ctrl;
     double ab = 0.0, at = 0.0;
     long long nb = 0;
+    // Group b's partial: the last two levels of its tree over the four
+    // waves' entries, (v0 + v2) + (v1 + v3).  Only the groups the step
+    // flagged are read: the others' partials are zeros, and adding a zero
+    // leaves these sums' bits unchanged (they start at +0.0, so they are
+    // never -0.0).
 #pragma unroll 4
     for (int b = t; b < p.nblk; b += kCtrlThreads) {
-        ab = ab + s.part_b[b];
-        at = at + s.part_t[b];
-        nb += s.part_bonds[b];
+        if ((s.pmask[b >> 5] >> (b & 31)) & 1u) {
+            const double4 vb = reinterpret_cast<const double4*>(s.part_b)[b];
+            const double4 vt = reinterpret_cast<const double4*>(s.part_t)[b];
+            ab = ab + ((vb.x + vb.z) + (vb.y + vb.w));
+            at = at + ((vt.x + vt.z) + (vt.y + vt.w));
+        }
     }
+    // bond counts: only on the steps that record a bond-count sample
+    const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
+    if (p.kind != RF_TEST_NONE && c0.step % dn == 0) {
+#pragma unroll 4
+        for (int b = t; b < p.nblk; b += kCtrlThreads) {
+            const int4 vn = reinterpret_cast<const int4*>(s.part_bonds)[b];
+            nb += (vn.x + vn.y) + (vn.z + vn.w);
+        }
+    }
+    __syncthreads();  // every flag read: clear them for the next step
+    for (int w = t; w < (p.nblk + 31) / 32; w += kCtrlThreads) s.pmask[w] = 0u;
     if (c0.halt) return;
     // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
     // one shuffle per level
@@ -1395,14 +1411,15 @@ hipError_t launch_clear_flags(Ctrl* ctrl, hipStream_t st) {
 }
 
 hipError_t launch_step(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st) {
-    hipLaunchKernelGGL(k_step, dim3((unsigned)p.nblk), dim3(kBlock), 0, st, s, p, ctrl);
+    hipLaunchKernelGGL(k_step, dim3((unsigned)(p.nblk * kWavesPerGroup)), dim3(kBlock), 0, st, s,
+                       p, ctrl);
     return hipGetLastError();
 }
 
 hipError_t launch_step_timed(const DevState& s, const DevParams& p, Ctrl* ctrl, hipStream_t st,
                              hipEvent_t start, hipEvent_t stop) {
-    hipExtLaunchKernelGGL(k_step, dim3((unsigned)p.nblk), dim3(kBlock), 0, st, start, stop, 0, s, p,
-                          ctrl);
+    hipExtLaunchKernelGGL(k_step, dim3((unsigned)(p.nblk * kWavesPerGroup)), dim3(kBlock), 0, st,
+                          start, stop, 0, s, p, ctrl);
     return hipGetLastError();
 }
 
@@ -1413,8 +1430,8 @@ hipError_t launch_controller(const DevState& s, const DevParams& p, Ctrl* ctrl, 
 
 hipError_t launch_step_batch(const BatchEntry* e, const int32_t* blk_sample,
                              const int32_t* blk_local, int32_t nblk_total, hipStream_t st) {
-    hipLaunchKernelGGL(k_step_batch, dim3((unsigned)nblk_total), dim3(kBlock), 0, st, e,
-                       blk_sample, blk_local);
+    hipLaunchKernelGGL(k_step_batch, dim3((unsigned)(nblk_total * kWavesPerGroup)), dim3(kBlock),
+                       0, st, e, blk_sample, blk_local);
     return hipGetLastError();
 }
 

@@ -370,10 +370,11 @@ rf_status rf_create(const rf_sample* s, const rf_material* mat, const rf_box* bo
     ALLOC(d.cfill, ncell);
     ALLOC(d.ctmp, n);
     ALLOC(d.sorted, n);
-    const int64_t nblk = (n + rf::kPerBlock - 1) / rf::kPerBlock;
-    ALLOC(d.part_b, nblk);
-    ALLOC(d.part_t, nblk);
-    ALLOC(d.part_bonds, nblk);
+    const int64_t nblk = (n + rf::kGroup - 1) / rf::kGroup;
+    ALLOC(d.part_b, nblk * rf::kWavesPerGroup);
+    ALLOC(d.part_t, nblk * rf::kWavesPerGroup);
+    ALLOC(d.part_bonds, nblk * rf::kWavesPerGroup);
+    ALLOC(d.pmask, (nblk + 31) / 32);
     ALLOC(c->ctrl, 1);
     ALLOC(c->dcount, 4);
 #undef ALLOC

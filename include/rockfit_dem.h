@@ -116,7 +116,7 @@ typedef struct rf_options {
     double skin;          /* candidate-list skin width (GPU, default 0.08 acut; halved
                              when a list overflows); never changes results */
     int32_t device;       /* HIP device ordinal                                */
-    int32_t block;        /* threads per block of the per-particle kernels     */
+    int32_t block;        /* ignored (the step kernel's workgroup is one wave) */
     void* stream;         /* hipStream_t the context runs on: one the caller
                              created on `device` (not the null stream), kept
                              by the caller and not destroyed by rf_destroy;
