@@ -11,11 +11,11 @@
 namespace rf {
 
 // Step kernel: one 8-lane group per particle, lane l of a group owning
-// slots, bonds and candidates l, l+8, l+16, l+24; one wave (8 particles) per
-// workgroup, so a wave's slot is refilled as soon as it ends.  The platen
-// forces are summed per group of 32 consecutive particles (a tree); wave w
-// of group b holds particles 32 b + w + 4 k, k = 0..7, which puts the
-// tree's first three levels inside the wave (see step_body).
+// slots, bonds and candidates l, l+8, l+16, l+24; one wave (8 consecutive
+// particles) per workgroup, so a wave's slot is refilled as soon as it ends.
+// The platen forces are summed per group of 32 consecutive particles: each
+// wave's 8 by a tree inside the wave, then the group's four waves by the
+// controller (see step_body).
 constexpr int kLanes = 8;
 constexpr int kBlock = 64;                  // threads of a step workgroup: one wave
 constexpr int kPerWave = kBlock / kLanes;   // 8 particles per wave

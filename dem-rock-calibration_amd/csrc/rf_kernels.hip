@@ -185,14 +185,14 @@ __device__ __forceinline__ void contact_eval(const DevParams& p, const double4& 
 // its result masked afterwards, so a wave waits on two memory round trips
 // before any force work.  Contacts gather in rounds of one slot per lane.
 // Bonds beyond the 8th and appends are rare paths.
-// `wave`: wave w = wave % 4 of particle group b = wave / 4, holding
-// particles 32 b + w + 4 k (k = 0..7, the wave's 8-lane groups).
+// `wave`: particles 8 wave + k (k = 0..7, the wave's 8-lane groups); wave w
+// = wave % 4 of the 32-particle group wave / 4.
 __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
                                           Ctrl* __restrict__ ctrl, const int wave) {
     const int l = threadIdx.x & (kLanes - 1);
     const int grp = threadIdx.x / kLanes;
-    const int64_t gbase = (int64_t)(wave / kWavesPerGroup) * kGroup + wave % kWavesPerGroup;
-    const int64_t i = gbase + (int64_t)kWavesPerGroup * grp;
+    const int64_t gbase = (int64_t)wave * kPerWave;
+    const int64_t i = gbase + grp;
     const bool valid = i < p.n;
     // The control block (written by the previous controller launch) and the
     // particle's own rows, which do not depend on it, are requested together;
@@ -329,7 +329,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const int32_t bjs = __shfl(pre1 ? bp1 : (int32_t)ic, from);
         const int bss = __shfl(pre1 ? bs1 : -1, from);
         if (from != lane) {
-            const int64_t io = gbase + (int64_t)kWavesPerGroup * (from / kLanes);
+            const int64_t io = gbase + from / kLanes;
             bjx = bjs;
             qbx = bss >= 0 ? io * B + kLanes + (from & (kLanes - 1)) : io * B;
             bsx = bss;
@@ -730,11 +730,10 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         if (c < nc && c < C && cs[t] >= 0 && ((prune >> cs[t]) & 1u)) s.cslot[ic * C + c] = -1;
     }
 
-    // Platen partials of the group's 32 particles, tree v[g] = v[g] + v[g+off],
-    // off = 16, 8, 4, 2, 1 (g = particle - 32 b).  The wave holds g = w + 4 k,
-    // so off = 16, 8, 4 pair its own groups k and k + 4, 2, 1 (lanes 8 k and
-    // 8 k + 32, 16, 8); lane 0 then holds v[w], and the controller adds the
-    // last two levels, (v[0] + v[2]) + (v[1] + v[3]).
+    // Platen partial of the wave's 8 particles, tree v[k] = v[k] + v[k+off],
+    // off = 4, 2, 1 (particle k's values in lane 8 k: shuffles by 32, 16 and
+    // 8 lanes); lane 0 then holds the wave's W, and the controller adds the
+    // group's four as (W0 + W2) + (W1 + W3).
 #pragma unroll
     for (int off = 32; off >= kLanes; off >>= 1) {
         gb = gb + __shfl_down(gb, off);
@@ -863,8 +862,8 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     const Ctrl c0 = *ctrl;
     double ab = 0.0, at = 0.0;
     long long nb = 0;
-    // Group b's partial: the last two levels of its tree over the four
-    // waves' entries, (v0 + v2) + (v1 + v3).  Only the groups the step
+    // Group b's partial: its four waves' entries as (W0 + W2) + (W1 + W3).
+    // Only the groups the step
     // flagged are read: the others' partials are zeros, and adding a zero
     // leaves these sums' bits unchanged (they start at +0.0, so they are
     // never -0.0).

@@ -462,10 +462,10 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 }
 
 // Platen forces and bond counts per 32 particles, in the step kernel's
-// order: the tree v[g] += v[g+off] with -0.0 past the last particle, its
-// levels off = 16, 8, 4 here, leaving v[0..3] in the four entries of the
-// group that the controller combines as the step kernel's waves; tangents
-// per direction through all five levels.
+// order: each run of 8 particles (one wave of the step kernel) by the tree
+// v[k] += v[k+off], off = 4, 2, 1, with -0.0 past the last particle, into
+// the group's four entries that the controller combines; tangents per
+// direction by the tree v[g] += v[g+off], off = 16..1.
 __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t, DevState s,
                                                        Ctrl* __restrict__ ctrl) {
     if (ctrl->halt) return;
@@ -477,15 +477,16 @@ __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t,
     double vb = in ? t.pp[2 * i] : -0.0, vt = in ? t.pp[2 * i + 1] : -0.0;
     int nb = in ? t.pnb[i] : 0;
 #pragma unroll
-    for (int off = 16; off >= kWavesPerGroup; off >>= 1) {
-        vb = vb + __shfl_down(vb, off, 32);
-        vt = vt + __shfl_down(vt, off, 32);
-        nb += __shfl_down(nb, off, 32);
+    for (int off = kPerWave / 2; off >= 1; off >>= 1) {
+        vb = vb + __shfl_down(vb, off, kPerWave);
+        vt = vt + __shfl_down(vt, off, kPerWave);
+        nb += __shfl_down(nb, off, kPerWave);
     }
-    if (g < kWavesPerGroup && grp < p.nblk) {
-        s.part_b[grp * kWavesPerGroup + g] = vb;
-        s.part_t[grp * kWavesPerGroup + g] = vt;
-        s.part_bonds[grp * kWavesPerGroup + g] = nb;
+    if (g % kPerWave == 0 && grp < p.nblk) {
+        const int64_t q = grp * kWavesPerGroup + g / kPerWave;
+        s.part_b[q] = vb;
+        s.part_t[q] = vt;
+        s.part_bonds[q] = nb;
         if (vb != 0.0 || vt != 0.0) atomicOr(&s.pmask[grp >> 5], 1u << (grp & 31));
     }
     // tangents only at the steps whose curve point the controller records

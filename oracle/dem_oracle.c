@@ -38,7 +38,7 @@
 #define PI 3.1415926535897932384626433832795 /* MC_pi, md_math_constants.h */
 #define DAMP (-1.8257)                       /* -1.8257 = -2*sqrt(5/6), md_calculateforcesdem3.cu:248 */
 #define TWO_THIRDS (2.0 / 3.0)               /* MCf_2d3 */
-#define RED_BLOCK 32                         /* particles per platen partial (one device block) */
+#define RED_BLOCK 32                         /* particles per platen partial (four device waves) */
 #define CTRL_T 1024                          /* threads of the final platen-force sum */
 #define LANES 8                              /* lanes sharing one particle on the device */
 
@@ -967,11 +967,12 @@ static void platen_particle(ora_ctx* c, int64_t i, double* sb, double* st) {
     *st = -s_t;
 }
 
-/* Deterministic platen-force sum, the same order as the device: per block
- * of RED_BLOCK = 32 consecutive particles a pairwise tree (v[t] += v[t+off],
- * off = 16..1; -0.0 for indices past n, as the reference's idle threads
- * store -sfb = -0.0), then CTRL_T = 1024 strided running sums over the block
- * partials and a 1024-wide tree.  The reference's FP32 64-thread tree and host
+/* Deterministic platen-force sum, the same order as the device: per group
+ * of RED_BLOCK = 32 consecutive particles (-0.0 for indices past n, as the
+ * reference's idle threads store -sfb = -0.0) each run of 8 by a pairwise
+ * tree (v[t] += v[t+off], off = 4, 2, 1) and the four runs as
+ * (W0 + W2) + (W1 + W3), then CTRL_T = 1024 strided running sums over the
+ * group partials and a 1024-wide tree.  The reference's FP32 64-thread tree and host
  * loop (md_compression_hm.cu:243-300, md_results.cu:56-69) sum the same terms
  * in another order. */
 static double tree_n(double* v, int w) {
@@ -990,6 +991,15 @@ static double ctrl_sum(const double* blk, int64_t nblk) {
     return tree_n(buf, CTRL_T);
 }
 
+/* A group of RED_BLOCK = 32 particles in storage order (-0.0 past the
+ * last): each run of 8 by the tree v[k] += v[k+off], off = 4, 2, 1, then the
+ * four runs as (W0 + W2) + (W1 + W3). */
+static double group_tree(double* v) {
+    double w[4];
+    for (int r = 0; r < 4; ++r) w[r] = tree_n(v + 8 * r, 8);
+    return (w[0] + w[2]) + (w[1] + w[3]);
+}
+
 static void reduce_partials(ora_ctx* c) {
     double buf[RED_BLOCK];
     for (int64_t b = 0; b < c->nblk; ++b) {
@@ -997,15 +1007,24 @@ static void reduce_partials(ora_ctx* c) {
             int64_t i = b * RED_BLOCK + t;
             buf[t] = i < c->n ? c->part_b[c->ord[i]] : -0.0;
         }
-        c->blk_b[b] = tree_n(buf, RED_BLOCK);
+        c->blk_b[b] = group_tree(buf);
         for (int t = 0; t < RED_BLOCK; ++t) {
             int64_t i = b * RED_BLOCK + t;
             buf[t] = i < c->n ? c->part_t[c->ord[i]] : -0.0;
         }
-        c->blk_t[b] = tree_n(buf, RED_BLOCK);
+        c->blk_t[b] = group_tree(buf);
     }
     c->sfl0 = ctrl_sum(c->blk_b, c->nblk);
     c->sfl1 = ctrl_sum(c->blk_t, c->nblk);
+}
+
+/* The last ora_forces' per-particle platen forces (bottom, top), by the
+ * caller's particle id. */
+int ora_get_particle_partials(const ora_ctx* c, double* bottom, double* top, int64_t cap) {
+    if (cap < c->n) return RF_ERR_ARG;
+    memcpy(bottom, c->part_b, c->n * sizeof(double));
+    memcpy(top, c->part_t, c->n * sizeof(double));
+    return RF_OK;
 }
 
 int ora_get_partials(const ora_ctx* c, double* bottom, double* top, int64_t cap) {

@@ -63,6 +63,7 @@ int ora_get_counters(const ora_ctx* c, rf_counters* out);
 int ora_get_state(const ora_ctx* c, rf_state_view* v);
 int ora_set_state(ora_ctx* c, const rf_state_view* v);
 int ora_get_partials(const ora_ctx* c, double* bottom, double* top, int64_t cap);
+int ora_get_particle_partials(const ora_ctx* c, double* bottom, double* top, int64_t cap);
 
 /* Lognormal per-bond strength draw (also restated by the product host code). */
 /* FIRE relaxation of a packing (fire_oracle.c); same arguments as

@@ -75,6 +75,7 @@ def lib():
     L.ora_get_state.argtypes = [vp, P(abi.StateView)]
     L.ora_set_state.argtypes = [vp, P(abi.StateView)]
     L.ora_get_partials.argtypes = [vp, vp, vp, C.c_int64]
+    L.ora_get_particle_partials.argtypes = [vp, vp, vp, C.c_int64]
     L.ora_bond_strength.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int32, C.c_double,
                                     C.c_double]
     L.ora_bond_strength.restype = C.c_double
@@ -240,6 +241,14 @@ class Oracle:
     def set_state(self, st):
         v = make_view(self.abi, st)
         lib().ora_set_state(self.h, C.byref(v))
+
+    def particle_partials(self):
+        """Per-particle platen forces (bottom, top) of the last forces call,
+        by the caller's particle id."""
+        b = np.zeros(self.n)
+        t = np.zeros(self.n)
+        lib().ora_get_particle_partials(self.h, _ptr(b), _ptr(t), self.n)
+        return b, t
 
     def partials(self):
         nb = (self.n + 31) // 32

@@ -187,3 +187,44 @@ def test_platen_sum_order():
         x.step(150)
     a, b = o.test_state(), g.test_state()
     assert a.sfl1 != 0.0 and abs(a.sfl1 - b.sfl1) <= 1e-12 * abs(a.sfl1)
+
+
+def _tree(v):
+    v = list(v)
+    off = len(v) // 2
+    while off >= 1:
+        for t in range(off):
+            v[t] = v[t] + v[t + off]
+        off //= 2
+    return v[0]
+
+
+def test_platen_group_tree():
+    """The group partials of the platen-force sum: per 32 particles in
+    storage order (-0.0 past the last), each run of 8 by the tree
+    v[k] += v[k+off], off = 4, 2, 1, then the four runs as
+    (W0 + W2) + (W1 + W3) -- the device's waves and controller.  Restated
+    in numpy float64 and compared bit for bit, with platens pressed into the
+    specimen so that many particles carry distinct forces; a plain 32-wide
+    tree gives different bits, so the check tells the orders apart."""
+    w = config.make_workload(n=1200, r_min=0.2, r_max=0.3, test="uniaxial", seed=6)
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"])
+    o.build_bonds()
+    cfg = w["cfg"]
+    cfg.size[2] -= 0.1  # both platens 0.05 mm inside the envelope
+    o.set_test(cfg)
+    o.call("ora_forces")
+    pb, pt = o.particle_partials()
+    gb, gt = o.partials()
+    order = o.order()
+    n = o.n
+    assert np.count_nonzero(pt) > 25 and np.count_nonzero(pb) > 25
+    differs = False
+    for per, got in ((pb, gb), (pt, gt)):
+        for g in range(len(got)):
+            v = [float(per[order[32 * g + j]]) if 32 * g + j < n else -0.0 for j in range(32)]
+            W = [_tree(v[8 * r:8 * r + 8]) for r in range(4)]
+            expect = (W[0] + W[2]) + (W[1] + W[3])
+            assert np.float64(got[g]).tobytes() == np.float64(expect).tobytes(), g
+            differs |= np.float64(_tree(v)).tobytes() != np.float64(expect).tobytes()
+    assert differs
