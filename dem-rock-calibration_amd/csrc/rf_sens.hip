@@ -35,7 +35,10 @@ using D = Dual<1>;
 
 namespace {
 
-constexpr int kSensBlock = 256;
+#ifndef RF_SENS_BLOCK
+#define RF_SENS_BLOCK 256
+#endif
+constexpr int kSensBlock = RF_SENS_BLOCK;  // threads per workgroup (whole waves)
 
 // Minimum waves per SIMD the register allocation must allow (1: no limit,
 // the kernel keeps everything in registers; 2 halves the registers at the
@@ -96,6 +99,97 @@ __device__ __forceinline__ MatT<D> mat_dual(const DevParams& p, const SensState&
     return m;
 }
 
+// Bond balancing.  At 20k spheres a window's kernel is a single round of
+// waves, so it lasts as long as its slowest lane, and a lane walks all of its
+// particle's bonds: the most bonded particle of a wave sets its time (12
+// bonds against a mean of 6.2 in the gradient workload).  Each direction
+// class of a wave (its lanes d, d + nl, ...) therefore shares the class's
+// bonds out: a lane evaluates the first `target` = ceil(class bonds / class
+// particles) positions of its own particle, and the positions past that go to
+// lanes of the class with fewer bonds, in the rounds those have left; their
+// contributions come back through LDS and are added by the owner afterwards,
+// in the owner's order, so every sum keeps its order.
+constexpr int kHelpCap = 32;  // positions handed out per class and wave
+constexpr int kSensWaves = kSensBlock / 64;
+
+// Inclusive scan over the particles of a direction class (lanes pw * nl + d).
+__device__ __forceinline__ int class_scan(int v, int pw, int nl) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, (unsigned)(off * nl));
+        if (pw >= off) v += u;
+    }
+    return v;
+}
+
+// Virtual lane visited idx-th: 0, 4, 2, 6, 1, 5, 3, 7.
+__device__ __forceinline__ int visit_lane(int idx) {
+    return ((idx & 1) << 2) | (idx & 2) | ((idx >> 2) & 1);
+}
+
+// Bonds of virtual lane l among n: l, l + 8, ... below n.
+__device__ __forceinline__ int lane_bonds(int l, int n) {
+    return n > l ? (n - 1 - l) / kLanes + 1 : 0;
+}
+
+// Bond at position q of a particle's visiting order.
+__device__ __forceinline__ int seq_bond(int q, int n) {
+    for (int idx = 0; idx < kLanes; ++idx) {
+        const int l = visit_lane(idx);
+        const int c = lane_bonds(l, n);
+        if (q < c) return l + kLanes * q;
+        q -= c;
+    }
+    return -1;
+}
+
+// The per-particle sum of the step kernel's order, fed one position at a
+// time: each virtual lane adds its entries from 0.0 in ascending order, and
+// the lane sums meet in the tree ((v0+v4)+(v2+v6)) + ((v1+v5)+(v3+v7)),
+// combined as the lanes complete (visiting order 0,4,2,6,1,5,3,7).
+struct TreeSum {
+    D hold[6], p1[6], q1[6], acc[6], tot[6];
+    int idx, rem, n;
+    __device__ __forceinline__ void init(int n_) {
+        n = n_;
+        idx = 0;
+        rem = lane_bonds(visit_lane(0), n);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) acc[u] = dconst<1>(0.0);
+    }
+    __device__ __forceinline__ void close() {
+        if (!(idx & 1)) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) hold[u] = acc[u];
+        } else if (!(idx & 2)) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) p1[u] = hold[u] + acc[u];
+        } else if (idx == 3) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) q1[u] = p1[u] + (hold[u] + acc[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) tot[u] = q1[u] + (p1[u] + (hold[u] + acc[u]));
+        }
+        ++idx;
+#pragma unroll
+        for (int u = 0; u < 6; ++u) acc[u] = dconst<1>(0.0);
+        rem = idx < kLanes ? lane_bonds(visit_lane(idx), n) : 0;
+    }
+    // the next position: a bond that is alive adds its contribution
+    __device__ __forceinline__ void add(bool alive, const D (&c6)[6]) {
+        while (rem == 0 && idx < kLanes) close();
+        if (alive) {
+#pragma unroll
+            for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c6[u];
+        }
+        --rem;
+    }
+    __device__ __forceinline__ void finish() {
+        while (idx < kLanes) close();
+    }
+};
+
 }  // namespace
 
 template <int MODE>
@@ -121,15 +215,81 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
     const MatT<D> m = mat_dual(p, t, dir);
     D pb = dconst<1>(-0.0), pt = dconst<1>(-0.0);
     int nb_out = 0;
+
+    // ---- bond balancing within each direction class (every lane of the wave)
+    const int64_t ic = valid ? i : 0;
+    const int wv = threadIdx.x >> 6;
+    const int64_t wave_base = ((int64_t)blockIdx.x * kSensWaves + wv) * ppw;
+    const uint32_t bnd0 = valid ? s.bnd[ic] : 0u;
+    const int nbd0 = valid ? s.nbond[ic] : 0;
+    const int nown = nbd0 < B ? nbd0 : B;
+    // the slots and partners of the first 16 bonds (rows of 32: 16-byte aligned)
+    const uint4 bsl = *reinterpret_cast<const uint4*>(s.bslot + ic * B);
+    int4 bpr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bpr[q] = reinterpret_cast<const int4*>(s.bpart + ic * B)[q];
+    auto slot_partner = [&](int b, int& ks, int64_t& j) {
+        if (b < 16) {
+            const uint32_t w = (b >> 2) == 0 ? bsl.x : (b >> 2) == 1 ? bsl.y
+                                                   : (b >> 2) == 2 ? bsl.z : bsl.w;
+            ks = (int)(int8_t)(uint8_t)(w >> (8 * (b & 3)));
+            const int4 e = (b >> 2) == 0 ? bpr[0] : (b >> 2) == 1 ? bpr[1]
+                                                  : (b >> 2) == 2 ? bpr[2] : bpr[3];
+            j = (b & 3) == 0 ? e.x : (b & 3) == 1 ? e.y : (b & 3) == 2 ? e.z : e.w;
+        } else {
+            ks = s.bslot[ic * B + b];
+            j = s.bpart[ic * B + b];
+        }
+    };
+    const uint32_t rec3_0 =
+        MODE == 2 && valid ? t.br_bits[(ls * p.n + ic) * kBrWords + 3] : 0u;
+    const int last = (ppw - 1) * nl + dir;  // the class's last particle lane
+    const int ntot = __shfl(class_scan(nown, pw, nl), last);
+    const int nval = __shfl(class_scan(valid ? 1 : 0, pw, nl), last);
+    int target = nval > 0 ? (ntot + nval - 1) / nval : 0;
+    int excess = nown > target ? nown - target : 0;
+    int freec = valid && target > nown ? target - nown : 0;
+    const int ex_in = class_scan(excess, pw, nl), fr_in = class_scan(freec, pw, nl);
+    int nhelp = __shfl(ex_in, last);  // the same in every class (same particles)
+    int ex_pre = ex_in - excess, fr_pre = fr_in - freec;
+    if (nhelp > kHelpCap) {  // too many to hand out: every lane walks its own bonds
+        int mx = nown;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) mx = max(mx, __shfl_xor(mx, off));
+        target = mx;
+        excess = freec = nhelp = ex_pre = fr_pre = 0;
+    }
+    __shared__ int32_t h_j[kSensWaves][kSensK][kHelpCap];
+    __shared__ int8_t h_ks[kSensWaves][kSensK][kHelpCap];
+    __shared__ uint8_t h_pw[kSensWaves][kSensK][kHelpCap];
+    __shared__ uint8_t h_b[kSensWaves][kSensK][kHelpCap];
+    __shared__ uint8_t h_flag[kSensWaves][kSensK][kHelpCap];  // 1: alive, 2: recorded break
+    __shared__ uint8_t h_brk[kSensWaves][kSensK][kHelpCap];
+    __shared__ double h_c6[kSensWaves][kSensK][kHelpCap][12];
+    for (int k = 0; k < excess; ++k) {  // this particle's positions past `target`
+        const int e = ex_pre + k;
+        const int b = seq_bond(target + k, nown);
+        int ks;
+        int64_t j;
+        slot_partner(b, ks, j);
+        const bool alive = ks >= 0 && ((bnd0 >> ks) & 1u);
+        h_j[wv][dir][e] = (int32_t)j;
+        h_ks[wv][dir][e] = (int8_t)ks;
+        h_pw[wv][dir][e] = (uint8_t)pw;
+        h_b[wv][dir][e] = (uint8_t)b;
+        h_flag[wv][dir][e] = (uint8_t)((alive ? 1u : 0u) |
+                                       (alive && ((rec3_0 >> ks) & 1u) ? 2u : 0u));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
     if (valid) {
         const double4* pos = s.pos[cur];
         const double4* vel = s.vel[cur];
         const double4* angv = s.angv[cur];
-        const Vec4<D> pi = load4(pos[i], t.tpos[cur], i, dir);
-        const Vec4<D> vi = load4(vel[i], t.tvel[cur], i, dir);
-        const Vec4<D> wi = load4(angv[i], t.tang[cur], i, dir);
-        const D ri = pi.w, mi = vi.w;
-        uint32_t occ = s.occ[i], bnd = s.bnd[i], otnz = s.otnz[i];
+        Vec4<D> pi = load4(pos[i], t.tpos[cur], i, dir);
+        Vec4<D> vi = load4(vel[i], t.tvel[cur], i, dir);
+        Vec4<D> wi = load4(angv[i], t.tang[cur], i, dir);
+        uint32_t occ = s.occ[i], bnd = bnd0, otnz = s.otnz[i];
         uint32_t* const lw = MODE != 0 ? t.br_bits + (ls * p.n + i) * kBrWords : nullptr;
         uint32_t rec[kBrWords] = {0u, 0u, 0u, 0u, 0u};  // the decisions (taken or replayed)
         double4 wrec = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -233,53 +393,65 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 }
             }
         };
-        // bonds in creation order; entry b goes to lane b % 8.  The slots
-        // and partners of the first 16 bonds are loaded up front (rows of
-        // 32 entries: 16-byte aligned).
-        const int nbd = s.nbond[i];
-        const uint4 bsl = *reinterpret_cast<const uint4*>(s.bslot + i * B);
-        int4 bpr[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) bpr[q] = reinterpret_cast<const int4*>(s.bpart + i * B)[q];
-        auto bond_lane = [&](int l, D (&acc)[6]) {
+        // Bonds in creation order, entry b of a particle in virtual lane b % 8
+        // (visiting order above).  Round r evaluates this particle's r-th
+        // position while r < target, then the class's handed-out positions
+        // assigned to this lane (their particle's state read from memory).
+        TreeSum ts;
+        ts.init(nown);
+        const int own_n = nown < target ? nown : target;
 #pragma unroll 1
-            for (int b = l; b < nbd && b < B; b += kLanes) {
-                int ks;
-                int64_t j;
-                if (b < 16) {
-                    const uint32_t w = (b >> 2) == 0 ? bsl.x : (b >> 2) == 1 ? bsl.y
-                                                           : (b >> 2) == 2 ? bsl.z : bsl.w;
-                    ks = (int)(int8_t)(uint8_t)(w >> (8 * (b & 3)));
-                    const int4 e = (b >> 2) == 0 ? bpr[0] : (b >> 2) == 1 ? bpr[1]
-                                                          : (b >> 2) == 2 ? bpr[2] : bpr[3];
-                    j = (b & 3) == 0 ? e.x : (b & 3) == 1 ? e.y : (b & 3) == 2 ? e.z : e.w;
-                } else {
-                    ks = s.bslot[i * B + b];
-                    j = s.bpart[i * B + b];
+        for (int r = 0; r < target; ++r) {
+            const bool own = r < own_n;
+            const int e = fr_pre + (r - own_n);
+            if (!own && !(r - own_n < freec && e < nhelp)) continue;
+            int64_t ip, j;
+            int b, ks;
+            bool alive;
+            uint32_t brk_in;
+            if (own) {
+                ip = i;
+                b = seq_bond(r, nown);
+                slot_partner(b, ks, j);
+                alive = ks >= 0 && ((bnd >> ks) & 1u);
+                brk_in = alive ? (rec[3] >> ks) & 1u : 0u;
+            } else {
+                ip = wave_base + h_pw[wv][dir][e];
+                b = h_b[wv][dir][e];
+                ks = h_ks[wv][dir][e];
+                j = h_j[wv][dir][e];
+                const uint32_t f = h_flag[wv][dir][e];
+                alive = (f & 1u) != 0u;
+                brk_in = (f >> 1) & 1u;
+            }
+            D c6[6];
+#pragma unroll
+            for (int u = 0; u < 6; ++u) c6[u] = dconst<1>(0.0);
+            bool brk = false;
+            if (alive) {
+                const int64_t qb = ip * B + b;
+                Vec4<D> pq = pi, vq = vi, wq = wi;
+                if (!own) {
+                    pq = load4(pos[ip], t.tpos[cur], ip, dir);
+                    vq = load4(vel[ip], t.tvel[cur], ip, dir);
+                    wq = load4(angv[ip], t.tang[cur], ip, dir);
                 }
-                if (ks < 0 || !((bnd >> ks) & 1u)) continue;
-                const int64_t qb = i * B + b;
                 const Vec4<D> pj = load4(pos[j], t.tpos[cur], j, dir);
                 const Vec4<D> vj = load4(vel[j], t.tvel[cur], j, dir);
                 const Vec4<D> wj = load4(angv[j], t.tang[cur], j, dir);
-                const BondRec r = s.brec[qb];
-                const double hv[12] = {r.pnx, r.pny, r.pnz, r.otx, r.oty, r.otz,
-                                       r.mnx, r.mny, r.mnz, r.mtx, r.mty, r.mtz};
+                const BondRec rr = s.brec[qb];
+                const double hv[12] = {rr.pnx, rr.pny, rr.pnz, rr.otx, rr.oty, rr.otz,
+                                       rr.mnx, rr.mny, rr.mnz, rr.mtx, rr.mty, rr.mtz};
                 double* tb = t.tbrec + (qb * kSensK + dir) * 12;
                 D hist[12];
 #pragma unroll
                 for (int u = 0; u < 12; ++u) hist[u] = mk(hv[u], tb[u]);
-                D c6[6], h[12];
+                D h[12];
                 SensBr<MODE> br;
-                br.in = ((rec[3] >> ks) & 1u) << kBrBreak;
-                const bool brk = bond_eval_t<D>(m, p, pi, vi, wi, pj, vj, wj, hist, r.invL0, r.L0,
-                                                r.sc, r.tc, c6, h, br);
-                if (MODE == 1) rec[3] |= (uint32_t)brk << ks;
-#pragma unroll
-                for (int u = 0; u < 6; ++u) acc[u] = acc[u] + c6[u];
-                if (brk) {
-                    bnd &= ~(1u << ks);
-                } else {
+                br.in = brk_in << kBrBreak;
+                brk = bond_eval_t<D>(m, p, pq, vq, wq, pj, vj, wj, hist, rr.invL0, rr.L0, rr.sc,
+                                     rr.tc, c6, h, br);
+                if (!brk) {
                     if (lead) {
                         BondRec& w = s.brec[qb];
                         w.pnx = h[0].v; w.pny = h[1].v; w.pnz = h[2].v;
@@ -291,9 +463,39 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                     for (int u = 0; u < 12; ++u) tb[u] = h[u].d[0];
                 }
             }
-        };
+            if (own) {
+                if (MODE == 1 && alive) rec[3] |= (uint32_t)brk << ks;
+                ts.add(alive, c6);
+                if (brk) bnd &= ~(1u << ks);
+            } else {
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    h_c6[wv][dir][e][2 * u] = c6[u].v;
+                    h_c6[wv][dir][e][2 * u + 1] = c6[u].d[0];
+                }
+                h_brk[wv][dir][e] = brk ? 1u : 0u;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int k = 0; k < excess; ++k) {  // positions evaluated by other lanes
+            const int e = ex_pre + k;
+            const bool alive = (h_flag[wv][dir][e] & 1u) != 0u;
+            const int ks = h_ks[wv][dir][e];
+            D c6[6];
+#pragma unroll
+            for (int u = 0; u < 6; ++u)
+                c6[u] = alive ? mk(h_c6[wv][dir][e][2 * u], h_c6[wv][dir][e][2 * u + 1])
+                              : dconst<1>(0.0);
+            const bool brk = alive && h_brk[wv][dir][e] != 0u;
+            if (MODE == 1 && alive) rec[3] |= (uint32_t)brk << ks;
+            ts.add(alive, c6);
+            if (brk) bnd &= ~(1u << ks);
+        }
+        ts.finish();
         D bsum[6];
-        tree_sum(bond_lane, bsum);
+#pragma unroll
+        for (int u = 0; u < 6; ++u) bsum[u] = ts.tot[u];
+        const D ri = pi.w, mi = vi.w;
         // contacts in slot order; slot k goes to lane k % 8
         const D wim = wi.x * wi.x + wi.y * wi.y + wi.z * wi.z;
         const bool roll = MODE == 2 ? (rec[4] & kBfRoll) != 0u : wim.v > 1e-18;
