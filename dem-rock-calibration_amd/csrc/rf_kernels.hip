@@ -1,7 +1,7 @@
 // rf_kernels.hip -- gfx950 kernels of the bonded-particle DEM loading step.
 //
-// One 8-lane group per particle (eight particles per wave64, 32 per
-// 256-thread block); lane l of a group owns slots, candidates and bonds
+// One 8-lane group per particle, eight consecutive particles per wave64 and
+// one wave per workgroup; lane l of a group owns slots, candidates and bonds
 // l, l+8, l+16, l+24.  In one launch a group
 //   1. prunes and appends its interaction row (slot lanes check the stored
 //      partners, candidate lanes look for new ones),
