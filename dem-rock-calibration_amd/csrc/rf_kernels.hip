@@ -123,13 +123,28 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 #ifndef RF_CAND_TAIL_LATE
 #define RF_CAND_TAIL_LATE 1
 #endif
+// Three waves per SIMD: the step's register budget is 168 VGPRs (the rows
+// and rebuild position are loaded late, the slot partners of a contact round
+// are read back from the row, and the slots 24..31 gather after the first
+// batch), which hides a third more of its memory latency than two waves.
 #ifndef RF_STEP_WAVES_PER_EU
-#define RF_STEP_WAVES_PER_EU 0
+#define RF_STEP_WAVES_PER_EU 3
 #endif
 #if RF_STEP_WAVES_PER_EU > 0
 #define RF_STEP_ATTR __attribute__((amdgpu_waves_per_eu(RF_STEP_WAVES_PER_EU)))
 #else
 #define RF_STEP_ATTR
+#endif
+// The batched step looks up its sample's arrays and constants per
+// workgroup: held to three waves per SIMD it spills 40 bytes per lane, and
+// measured 3 % slower than with the registers of two waves.
+#ifndef RF_BATCH_WAVES_PER_EU
+#define RF_BATCH_WAVES_PER_EU 0
+#endif
+#if RF_BATCH_WAVES_PER_EU > 0
+#define RF_BATCH_ATTR __attribute__((amdgpu_waves_per_eu(RF_BATCH_WAVES_PER_EU)))
+#else
+#define RF_BATCH_ATTR
 #endif
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup w runs on XCD
@@ -146,6 +161,13 @@ __device__ __forceinline__ int xcd_block(int w, int nblk) {
     (void)nblk;
     return w;
 #endif
+}
+
+// x, y, z of a double4 (the w component left unread: 0.0).
+__device__ __forceinline__ double4 ld3(const double4* a, int64_t q) {
+    const double* d = reinterpret_cast<const double*>(a + q);
+    const double2 xy = *reinterpret_cast<const double2*>(d);
+    return make_double4(xy.x, xy.y, d[2], 0.0);
 }
 
 __device__ __forceinline__ bool bond_eval(const DevParams& p, const double4& pi, const double4& vi,
@@ -178,12 +200,17 @@ __device__ __forceinline__ void contact_eval(const DevParams& p, const double4& 
 // l + 8t (t = 0..3).  Per-particle sums: each lane adds its own entries in
 // ascending order, then tree8.
 //
-// Memory schedule: every per-particle load (state, masks, slot row,
-// candidate row, first bond index) is issued first; then every gather that
-// depends only on those (partners for the prune test, absent candidates,
-// first bond's partner and record) is issued at once with clamped indices and
-// its result masked afterwards, so a wave waits on two memory round trips
-// before any force work.  Contacts gather in rounds of one slot per lane.
+// Memory schedule: every per-particle row (masks, slot row, candidate row,
+// first bond index) is issued first, then the particle's state; then every
+// gather that depends only on those (partners of slots 0..23 for the prune
+// test, absent candidates, first bond's partner and record) is issued at
+// once with clamped indices and its result masked afterwards, so a wave
+// waits on two memory round trips before any force work.  What that batch
+// leaves out keeps the step within the 168 registers of three waves per
+// SIMD: the partners of slots 24..31 (gathered after it, in the waves that
+// have any), (dt/m, dt/I) and the rebuild position (read at the
+// integration), the partner of a contact slot (read back from the row in
+// its contact round).  Contacts gather in rounds of one slot per lane.
 // Bonds beyond the 8th and appends are rare paths.
 // `wave`: particles 8 wave + k (k = 0..7, the wave's 8-lane groups); wave w
 // = wave % 4 of the 32-particle group wave / 4.
@@ -203,9 +230,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const int S = p.S, C = p.C, B = p.B;
     const int64_t ic = valid ? i : 0;  // clamped index for the inactive groups' loads
 
-    // ---- stage 1: per-particle loads
-    const double2 pk = s.pk[ic];
-    const double4 rb = s.rbuild[ic];
+    // ---- stage 1: per-particle loads ((dt/m, dt/I) and the rebuild position
+    // are read at the integration, where they are used)
     const uint32_t occ_raw = s.occ[ic], bnd_raw = s.bnd[ic], otnz_raw = s.otnz[ic];
     const int nc_raw = s.ncand[ic], nbd_raw = s.nbond[ic];
     const uint32_t occ_in = valid ? occ_raw : 0u;
@@ -273,7 +299,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const double4* __restrict__ angv = cur ? s.angv[1] : s.angv[0];
     const double4 pi = pos[ic];
     const double4 vi = vel[ic];
-    const double4 wi = angv[ic];
+    const double4 wi = ld3(angv, ic);
 
     // ---- stage 2: gathers (clamped to particle i where not needed)
     double4 pp[kPerLane];
@@ -283,8 +309,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const int k = l + kLanes * t;
         const bool had = k < S && ((occ_in >> k) & 1u);
 #if RF_CAND_TAIL_LATE
-        if (t == kPerLane - 1 && !hi_slots)
-            pp[t] = pi;  // no partner in these slots anywhere in the wave
+        if (t == kPerLane - 1)
+            pp[t] = pi;  // gathered after this batch, in the waves that need it
         else
 #endif
             pp[t] = pos[had ? jk[t] : (int32_t)ic];
@@ -336,8 +362,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         }
     }
     const double4 pb0 = pos[bjx];
-    const double4 vb0 = vel[bjx];
-    const double4 wb0 = angv[bjx];
+    const double4 vb0 = ld3(vel, bjx);
+    const double4 wb0 = ld3(angv, bjx);
     double2 rec0[8];
     {
         const double2* rp = reinterpret_cast<const double2*>(s.brec + qbx);
@@ -356,6 +382,15 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     for (int t = 0; t < kPerLane; ++t) {
         const int k = l + kLanes * t;
         const bool had = k < S && ((occ_in >> k) & 1u);
+#if RF_CAND_TAIL_LATE
+        // Slots 24..31 hold partners only in particles with many partners or
+        // a fragmented row (a few per cent of waves): their gather comes
+        // here, in those waves only.
+        if (t == kPerLane - 1) {
+            if (!hi_slots) break;  // wave-uniform
+            pp[t] = pos[had ? jk[t] : (int32_t)ic];
+        }
+#endif
         const double dx = pp[t].x - pi.x, dy = pp[t].y - pi.y, dz = pp[t].z - pi.z;
         const double rr = dx * dx + dy * dy + dz * dz;
         const double rs = ri + pp[t].w;
@@ -398,6 +433,13 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
             nc_out = C;
             if (l == 0) ctrl->rebuild = 1;
         }
+        // Candidates whose slot was pruned are absent again.  (The entries
+        // the append below links to a slot are others: they had no slot.)
+#pragma unroll
+        for (int t = 0; t < kPerLane; ++t) {
+            const int c = l + kLanes * t;
+            if (c < nc && c < C && cs[t] >= 0 && ((prune >> cs[t]) & 1u)) s.cslot[ic * C + c] = -1;
+        }
     }
 
     // ---- 1b. append (d_ReConstructInteractionListbpm)
@@ -406,7 +448,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const double sx = copysign(1.0, pi.x - (fx + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
     const double sy = copysign(1.0, pi.y - (fy + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
     const double sz = copysign(1.0, pi.z - (fz + 0.5) * p.cell) > 0.0 ? 1.0 : -1.0;
-    uint64_t key[kPerLane] = {0, 0, 0, 0};
+    uint32_t spk = 0u;   // 4 bits per entry t: the stencil position of a new pair
     uint32_t nw_l = 0u;  // bit t: candidate l + 8t is a new pair
     uint32_t npc = 0u;   // bit t: the new pair may touch
     uint32_t crem = cab;
@@ -426,7 +468,7 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const double rs = ri + pj.w;
 #pragma unroll
         for (int u = 0; u < kPerLane; ++u)
-            if (u == tc && nw) key[u] = (sp << 32) | (uint32_t)s.orig[jc];
+            if (u == tc && nw) spk |= (uint32_t)sp << (4 * u);
         nw_l |= (uint32_t)nw << tc;
         npc |= (uint32_t)(nw & (rr < rs * rs * 1.000001)) << tc;
         crem &= crem - 1u;  // clears bit tc
@@ -450,6 +492,13 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         // give them the free slots in the reference's fill order.
         int rank[kPerLane] = {0, 0, 0, 0};
         int total = 0;
+        // visiting-order key of each new pair: (stencil position, caller id)
+        uint64_t key[kPerLane];
+#pragma unroll
+        for (int t = 0; t < kPerLane; ++t)
+            key[t] = ((nw_l >> t) & 1u)
+                         ? ((uint64_t)((spk >> (4 * t)) & 15u) << 32) | (uint32_t)s.orig[cj[t]]
+                         : 0ull;
 #pragma unroll
         for (int t2 = 0; t2 < kPerLane; ++t2) {
             uint32_t m = nbt[t2];
@@ -479,24 +528,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         }
         const uint32_t added = grp_or(add_l);
         pc_l |= grp_or(pcadd_l);
-        // hand each filled slot's partner to its owner lane (slot % 8)
-#pragma unroll
-        for (int t2 = 0; t2 < kPerLane; ++t2) {
-            uint32_t m = nbt[t2];
-            while (m) {
-                const int L = __builtin_ctz(m);
-                m &= m - 1u;
-                const int rk = __shfl(rank[t2], L, kLanes);
-                const int32_t j = __shfl(cj[t2], L, kLanes);
-                const int sl = free_slot(freem, rk, q0);
-                if (sl >= 0 && (sl & (kLanes - 1)) == l) {
-                    const int tt = sl / kLanes;
-#pragma unroll
-                    for (int t = 0; t < kPerLane; ++t)
-                        if (t == tt) jk[t] = j;
-                }
-            }
-        }
+        // the new row entries are read back by the contact rounds below
+        __threadfence_block();
         occ |= added;
     }
     uint32_t pcm = 0u;
@@ -559,8 +592,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const bool live = bks >= 0 && ((bnd >> bks) & 1u);
         const int32_t bj = live ? bjr : (int32_t)ic;
         const double4 pj = pos[bj];
-        const double4 vj = vel[bj];
-        const double4 wj = angv[bj];
+        const double4 vj = ld3(vel, bj);
+        const double4 wj = ld3(angv, bj);
         double2 rec[8];
         const double2* rp = reinterpret_cast<const double2*>(s.brec + (live ? qb : ic * B));
 #pragma unroll
@@ -600,15 +633,14 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         const int t = mine ? __builtin_ctz(mine) : kPerLane;
         mine &= mine - 1u;
         const bool act = t < kPerLane;
-        int32_t j = (int32_t)ic;
-#pragma unroll
-        for (int u = 0; u < kPerLane; ++u)
-            if (u == t) j = jk[u];
         const int k = l + kLanes * (act ? t : 0);
         const int64_t qs = ic * S + k;
+        // the slot's partner, read back from the row (which holds this
+        // step's appends) rather than kept in registers since stage 1
+        const int32_t j = act ? s.nbr[qs] : (int32_t)ic;
         const double4 pj = pos[j];
         const double4 vj = vel[j];
-        const double4 wj = angv[j];
+        const double4 wj = ld3(angv, j);
         const double4 ot = s.ot[qs];
         const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
         const bool ov = act && sqrt(dx * dx + dy * dy + dz * dz) < ri + pj.w;
@@ -641,6 +673,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     double gb = -0.0, gt = -0.0;  // the particle's platen forces (lane 0; -0.0 past the last)
     int gn = 0;                   // and its bonds
     if (valid && l == 0) {
+        const double2 pk = s.pk[ic];
+        const double4 rb = s.rbuild[ic];
         const double mi = vi.w;
         double part_b = -0.0, part_t = -0.0;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0;
@@ -723,12 +757,6 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         gt = part_t;
         gn = __popc(bnd);
     }
-    // candidates whose slot was pruned are absent again
-#pragma unroll
-    for (int t = 0; t < kPerLane; ++t) {
-        const int c = l + kLanes * t;
-        if (c < nc && c < C && cs[t] >= 0 && ((prune >> cs[t]) & 1u)) s.cslot[ic * C + c] = -1;
-    }
 
     // Platen partial of the wave's 8 particles, tree v[k] = v[k] + v[k+off],
     // off = 4, 2, 1 (particle k's values in lane 8 k: shuffles by 32, 16 and
@@ -758,7 +786,7 @@ __global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step(DevState s, DevPar
 // blk_local[G] of sample blk_sample[G] (its four waves are L = 4 G .. 4 G + 3),
 // so every sample runs exactly as it would alone; the host chooses the order
 // of the table (the batch autotuner), which never changes a result.
-__global__ __launch_bounds__(kBlock) RF_STEP_ATTR void k_step_batch(
+__global__ __launch_bounds__(kBlock) RF_BATCH_ATTR void k_step_batch(
     const BatchEntry* __restrict__ e, const int32_t* __restrict__ blk_sample,
     const int32_t* __restrict__ blk_local) {
     const int L = xcd_block((int)blockIdx.x, (int)gridDim.x);
