@@ -877,8 +877,16 @@ __device__ __forceinline__ void ctrl_logic(const DevState& s, const DevParams& p
     *ctrl = c;
 }
 
+// Work a controller variant adds between the partial sums' loads and their
+// tree (every thread of the block calls it, with the step's control block).
+struct NoMid {
+    __device__ __forceinline__ void operator()(const Ctrl&) const {}
+};
+
+template <class Mid = NoMid>
 __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
-                                          Ctrl* __restrict__ ctrl, const VwLog* lg = nullptr) {
+                                          Ctrl* __restrict__ ctrl, const VwLog* lg = nullptr,
+                                          const Mid& mid = Mid()) {
     __shared__ double sb[kCtrlThreads];
     __shared__ double stt[kCtrlThreads];
     __shared__ long long sn[kCtrlThreads];
@@ -916,6 +924,7 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     __syncthreads();  // every flag read: clear them for the next step
     for (int w = t; w < (p.nblk + 31) / 32; w += kCtrlThreads) s.pmask[w] = 0u;
     if (c0.halt) return;
+    mid(c0);
     // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
     // one shuffle per level
     sb[t] = ab;
@@ -951,55 +960,58 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevPara
 // Tangents of the platen forces where the value controller records a curve
 // point: per direction the block partials summed strided over the block's
 // threads and by the tree through LDS and shuffles, stored beside the point.
-// Reads the step and curve index before the value controller advances them.
-__device__ __forceinline__ void sens_ctrl_body(const DevParams& p, const SensState& t,
-                                               Ctrl* __restrict__ ctrl) {
-    __shared__ double tb[kCtrlThreads], tt[kCtrlThreads];
-    if (ctrl->halt || p.kind == RF_TEST_NONE) return;
-    const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
-    const int64_t step = ctrl->step, save = ctrl->stepsave;
-    if (step % dn != 0 || save >= p.curve_cap) return;  // uniform
-    const int tid = threadIdx.x;
-    for (int k = 0; k < t.nl; ++k) {
-        double ab = 0.0, at = 0.0;
-        for (int b = tid; b < p.nblk; b += kCtrlThreads) {
-            ab = ab + t.tpart_b[k * p.nblk + b];
-            at = at + t.tpart_t[k * p.nblk + b];
-        }
-        tb[tid] = ab;
-        tt[tid] = at;
-        __syncthreads();
-        for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
-            if (tid < off) {
-                tb[tid] = tb[tid] + tb[tid + off];
-                tt[tid] = tt[tid] + tt[tid + off];
+// Runs inside the value controller, after its partial sums' loads and
+// before it advances the step and curve index.
+struct SensTangents {
+    const DevParams& p;
+    const SensState& t;
+    __device__ __forceinline__ void operator()(const Ctrl& c0) const {
+        __shared__ double tb[kCtrlThreads], tt[kCtrlThreads];
+        if (p.kind == RF_TEST_NONE) return;
+        const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
+        const int64_t save = c0.stepsave;
+        if (c0.step % dn != 0 || save >= p.curve_cap) return;  // uniform
+        const int tid = threadIdx.x;
+        for (int k = 0; k < t.nl; ++k) {
+            double ab = 0.0, at = 0.0;
+            for (int b = tid; b < p.nblk; b += kCtrlThreads) {
+                ab = ab + t.tpart_b[k * p.nblk + b];
+                at = at + t.tpart_t[k * p.nblk + b];
+            }
+            tb[tid] = ab;
+            tt[tid] = at;
+            __syncthreads();
+            for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
+                if (tid < off) {
+                    tb[tid] = tb[tid] + tb[tid + off];
+                    tt[tid] = tt[tid] + tt[tid + off];
+                }
+                __syncthreads();
+            }
+            if (tid < 64) {
+                double vb = tb[tid], vt = tt[tid];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    vb = vb + __shfl_down(vb, off);
+                    vt = vt + __shfl_down(vt, off);
+                }
+                if (tid == 0) {
+                    t.tcurve[(save * kSensK + k) * 2] = vb * p.inv_area;
+                    t.tcurve[(save * kSensK + k) * 2 + 1] = vt * p.inv_area;
+                }
             }
             __syncthreads();
         }
-        if (tid < 64) {
-            double vb = tb[tid], vt = tt[tid];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                vb = vb + __shfl_down(vb, off);
-                vt = vt + __shfl_down(vt, off);
-            }
-            if (tid == 0) {
-                t.tcurve[(save * kSensK + k) * 2] = vb * p.inv_area;
-                t.tcurve[(save * kSensK + k) * 2 + 1] = vt * p.inv_area;
-            }
-        }
-        __syncthreads();
     }
-}
+};
 
-// Controller of a sensitivity step: the tangents of a curve point, then the
-// value controller.
+// Controller of a sensitivity step: the value controller with the
+// tangents of a curve point.
 __global__ __launch_bounds__(kCtrlThreads) void k_controller_sens(DevState s, DevParams p,
                                                                   SensState t,
                                                                   Ctrl* __restrict__ ctrl) {
-    sens_ctrl_body(p, t, ctrl);
     const VwLog lg{t.br_mode, 0, t.br_step0, t.br_steps, t.br_vw};
-    ctrl_body(s, p, ctrl, &lg);
+    ctrl_body(s, p, ctrl, &lg, SensTangents{p, t});
 }
 
 // One block per sample.

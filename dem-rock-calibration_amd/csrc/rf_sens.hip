@@ -670,7 +670,10 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 // direction by the tree v[g] += v[g+off], off = 16..1.
 __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t, DevState s,
                                                        Ctrl* __restrict__ ctrl) {
-    if (ctrl->halt) return;
+    // the values are read together with the control block; a halted step
+    // writes nothing
+    const int32_t halt = ctrl->halt;
+    const int64_t step = ctrl->step, save = ctrl->stepsave;
     const int tid = threadIdx.x, g = tid & 31;  // one group per 32 lanes
     const int64_t grp = (int64_t)blockIdx.x * 8 + (tid >> 5);
     const int64_t i = grp * 32 + g;
@@ -678,6 +681,7 @@ __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t,
     const bool head = g == 0 && grp < p.nblk;
     double vb = in ? t.pp[2 * i] : -0.0, vt = in ? t.pp[2 * i + 1] : -0.0;
     int nb = in ? t.pnb[i] : 0;
+    if (halt) return;
 #pragma unroll
     for (int off = kPerWave / 2; off >= 1; off >>= 1) {
         vb = vb + __shfl_down(vb, off, kPerWave);
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t,
     }
     // tangents only at the steps whose curve point the controller records
     const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
-    if (p.kind == RF_TEST_NONE || ctrl->step % dn != 0 || ctrl->stepsave >= p.curve_cap) return;
+    if (p.kind == RF_TEST_NONE || step % dn != 0 || save >= p.curve_cap) return;
     for (int k = 0; k < t.nl; ++k) {
         double db = in ? t.tpp[(i * kSensK + k) * 2] : -0.0;
         double dt_ = in ? t.tpp[(i * kSensK + k) * 2 + 1] : -0.0;
