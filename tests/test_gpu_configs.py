@@ -1,6 +1,8 @@
 """The loading configurations through the HIP path, against the CPU oracle.
 
 C1: 5k-sphere cylinder, strain rate 0.1/s, 2000 loading steps.
+C2: the 100k-sphere benchmark sample, platens at 2 m/s for 2000 steps
+    (bonds break, pairs touch).
 C3: 50k-sphere Brazilian disc, platens at 0.02 m/s, 500 steps.
 C4: 64 samples (N log-uniform in [2k, 40k], Latin-hypercube material) as one
     batch for 200 steps: every sample equal to its own single run, two of
@@ -54,6 +56,24 @@ def test_c1_uniaxial_5k_strain_rate():
     assert ts.step == 2000 and ts.stress > 0.0  # the platens load the specimen
     c = g.counters()
     assert c.bonds > 4 * c.n
+
+
+@pytest.mark.timeout(300)
+def test_c2_uniaxial_100k_breaks_and_contacts():
+    """C2's own sample (100k spheres from the benchmark's generator, stored in
+    spatial order) with its platens driven at 2 m/s, so that within 2000
+    steps about 300 bonds break next to the platens and unbonded pairs
+    touch: every chunk bitwise against the oracle."""
+    print("case C2 100k uniaxial, platens at 2 m/s", flush=True)
+    w = config.config_workload("C2")
+    w["cfg"] = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=-2.0, dnsave=50)
+    g, o = _run_pair(w, (1, 999, 1000), "C2")
+    c = g.counters()
+    counts = g.bond_counts()
+    print(f"C2: bond count {counts[0]} -> {counts[-1]}, contacts {c.contacts}, "
+          f"rebuilds {c.rebuilds}", flush=True)
+    assert counts[-1] < counts[0]  # bonds broke
+    assert c.contacts > 0          # unbonded pairs touch
 
 
 @pytest.mark.timeout(300)
