@@ -3,6 +3,8 @@
 C1: 5k-sphere cylinder, strain rate 0.1/s, 2000 loading steps.
 C2: the 100k-sphere benchmark sample, platens at 2 m/s for 2000 steps
     (bonds break, pairs touch).
+C1 whole test: the same sample and material loaded at 1 m/s through
+    rf_run_uniaxial to its end, macroscopic results against the oracle.
 C3: 50k-sphere Brazilian disc, platens at 0.02 m/s, 500 steps.
 C4: 64 samples (N log-uniform in [2k, 40k], Latin-hypercube material) as one
     batch for 200 steps: every sample equal to its own single run, two of
@@ -74,6 +76,42 @@ def test_c2_uniaxial_100k_breaks_and_contacts():
           f"rebuilds {c.rebuilds}", flush=True)
     assert counts[-1] < counts[0]  # bonds broke
     assert c.contacts > 0          # unbonded pairs touch
+
+
+@pytest.mark.timeout(300)
+def test_c1_whole_test_macro_parity():
+    """C1's sample and material in a whole uniaxial test through the drop-in
+    entry point (rf_run_uniaxial), platens at 1 m/s, loading flagged above
+    2 MPa, until the fracture criterion or 3 % strain ends it, against the
+    oracle run to the same end: peak stress (UCS), peak strain, secant
+    modulus, linearity deviation, destruction, step count, curve and
+    bond-count trace identical, so their error is 0."""
+    print("case C1 whole uniaxial test, platens at 1 m/s", flush=True)
+    w = config.config_workload("C1")
+    cfg = config.uniaxial_cfg(w["xyz"], w["r"], v_load_mps=-1.0, min_stress_mpa=2.0,
+                              max_strain=0.03, dnsave=20, max_steps=100000)
+    g = Context(w["xyz"], w["r"], w["mat"], w["box"])
+    o = pyoracle.Oracle(w["xyz"], w["r"], w["mat"], w["box"], threads=THREADS)
+    g.build_bonds()
+    o.build_bonds()
+    res = g.run_uniaxial(cfg)
+    o.set_test(cfg)
+    o.step(100000)
+    o.post()
+    t = o.test_state()
+    print(f"C1: peak {res.peak_stress * 1e3:.4f} MPa at strain {res.peak_strain:.6f}, "
+          f"secant {res.secant_modulus:.4f} GPa, LD {res.linearity_deviation:.6f}, "
+          f"destruction {res.destruction:.6f}, fracture {res.fracture}, {res.steps} steps",
+          flush=True)
+    assert t.done == 1 and res.steps == t.step
+    assert res.fracture == t.fracture
+    assert res.peak_stress == t.peak_stress and res.peak_strain == t.peak_strain
+    assert res.secant_modulus == t.peak_stress / t.peak_strain
+    assert res.linearity_deviation == t.linearity_deviation
+    assert res.destruction == t.destruction and res.destruction > 0.0
+    np.testing.assert_array_equal(g.curve(), o.curve())
+    np.testing.assert_array_equal(g.bond_counts(), o.bond_counts())
+    compare_test_states(g.test_state(), t, what="end of the C1 test")
 
 
 @pytest.mark.timeout(300)
