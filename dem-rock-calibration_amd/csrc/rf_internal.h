@@ -62,7 +62,9 @@ struct DevState {
     double2* pk;       // (dt/m_i, dt/I_i)
     double4* F;        // last step's force  (x, y, z, 0)
     double4* M;        // last step's moment
-    double4* rbuild;   // positions at the last candidate rebuild
+    // position at the last candidate rebuild (x, y, z rounded to float) and
+    // the squared displacement that asks for the next one (see rebuild_mark)
+    float4* rbuild;
     // slot table
     int32_t* nbr;      // partner of slot k (valid where the occupancy bit is set)
     double4* ot;       // contact tangential overlap (x, y, z, 0)
@@ -114,6 +116,27 @@ struct DevParams {
     int64_t max_steps, curve_cap;
     int64_t zero_idx;    // storage index of the caller's particle 0
 };
+
+// Rebuild test.  A candidate rebuild is due once a particle has moved more
+// than skin/2 since the last one.  Only the timing of rebuilds depends on
+// this test, never a result, so the mark is kept in single precision with a
+// conservative threshold: x, y, z rounded to float (each off by at most
+// 2^-24 |x|), and the squared threshold (skin/2 - eta)^2 rounded down, with
+// eta = 2^-22 (|x| + |y| + |z|) four times the largest rounding of the mark.
+// A particle that moved more than skin/2 therefore always asks for the
+// rebuild; one that moved a little less may ask a step early.
+__device__ __forceinline__ float4 rebuild_mark(const DevParams& p, const double4& x) {
+    const double eta = 0x1p-22 * ((fabs(x.x) + fabs(x.y)) + fabs(x.z));
+    const double th = sqrt(p.skin_half2) - eta;
+    const double th2 = th > 0.0 ? th * th : 0.0;
+    return make_float4(__double2float_rn(x.x), __double2float_rn(x.y), __double2float_rn(x.z),
+                       __double2float_rd(th2));
+}
+
+__device__ __forceinline__ bool rebuild_due(const float4& m, double x, double y, double z) {
+    const double ex = x - (double)m.x, ey = y - (double)m.y, ez = z - (double)m.z;
+    return ex * ex + ey * ey + ez * ez > (double)m.w;
+}
 
 // Device control block: written by the controller kernel once per step.
 struct Ctrl {

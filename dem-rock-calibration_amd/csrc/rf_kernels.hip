@@ -230,8 +230,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     const int S = p.S, C = p.C, B = p.B;
     const int64_t ic = valid ? i : 0;  // clamped index for the inactive groups' loads
 
-    // ---- stage 1: per-particle loads ((dt/m, dt/I) and the rebuild position
-    // are read at the integration, where they are used)
+    // ---- stage 1: per-particle loads (the rebuild mark is read at the
+    // integration, where it is used)
     const uint32_t occ_raw = s.occ[ic], bnd_raw = s.bnd[ic], otnz_raw = s.otnz[ic];
     const int nc_raw = s.ncand[ic], nbd_raw = s.nbond[ic];
     const uint32_t occ_in = valid ? occ_raw : 0u;
@@ -673,9 +673,12 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     double gb = -0.0, gt = -0.0;  // the particle's platen forces (lane 0; -0.0 past the last)
     int gn = 0;                   // and its bonds
     if (valid && l == 0) {
-        const double2 pk = s.pk[ic];
-        const double4 rb = s.rbuild[ic];
+        const float4 rb = s.rbuild[ic];  // the rebuild mark, used last
         const double mi = vi.w;
+        // (dt/m, dt/I) as the context formed them at creation, from the mass
+        // and radius the particle carries: the same IEEE operations, so the
+        // same bits as reading them back, without the memory round trip
+        const double2 pk = make_double2((1.0 / mi) * p.dt, p.dt / (0.4 * mi * ri * ri));
         double part_b = -0.0, part_t = -0.0;
         double Fx = 0.0, Fy = 0.0, Fz = 0.0, Mx = 0.0, My = 0.0, Mz = 0.0;
         Fx += Fbx; Fy += Fby; Fz += Fbz;
@@ -751,8 +754,8 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
         // a particle that has just been moved to the hidden point changes the
         // lists; one that sits there already (and stays: F = V = 0) does not
         const bool was_hidden = (pi.x == p.hid[0]) & (pi.y == p.hid[1]) & (pi.z == p.hid[2]);
-        const double ex = nrx - rb.x, ey = nry - rb.y, ez = nrz - rb.z;
-        if ((out & !was_hidden) || ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
+        if ((out & !was_hidden) || (!was_hidden && rebuild_due(rb, nrx, nry, nrz)))
+            ctrl->rebuild = 1;
         gb = part_b;
         gt = part_t;
         gn = __popc(bnd);
@@ -1333,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_build_candidates(DevState s, DevParams 
     if (valid && l == 0) {
         if (total > C) atomicOr(&ctrl->err, kErrCandOverflow);
         s.ncand[i] = total < C ? total : (int32_t)C;
-        s.rbuild[i] = pi;
+        s.rbuild[i] = rebuild_mark(p, pi);
     }
 }
 

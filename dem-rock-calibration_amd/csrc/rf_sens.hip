@@ -646,9 +646,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 for (int u = 0; u < kBrWords; ++u) lw[u] = rec[u];
                 t.br_w[ls * p.n + i] = wrec;
             }
-            const double4 rb = s.rbuild[i];
-            const double ex = nrx.v - rb.x, ey = nry.v - rb.y, ez = nrz.v - rb.z;
-            if (ex * ex + ey * ey + ez * ez > p.skin_half2) ctrl->rebuild = 1;
+            if (rebuild_due(s.rbuild[i], nrx.v, nry.v, nrz.v)) ctrl->rebuild = 1;
         }
         nb_out = __popc(bnd);
     }
