@@ -120,13 +120,16 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 #ifndef RF_BOND_SPREAD
 #define RF_BOND_SPREAD 1
 #endif
+// Entries that few particles use (candidates 8..31, partners in slots
+// 24..31) are loaded after the first batch, only in the waves that have any.
 #ifndef RF_CAND_TAIL_LATE
 #define RF_CAND_TAIL_LATE 1
 #endif
-// Three waves per SIMD: the step's register budget is 168 VGPRs (the rows
-// and rebuild position are loaded late, the slot partners of a contact round
-// are read back from the row, and the slots 24..31 gather after the first
-// batch), which hides a third more of its memory latency than two waves.
+// Three waves per SIMD: the step's register budget is 168 VGPRs (the
+// rebuild mark is loaded at the integration, the slot partners of a contact
+// round are read back from the row, and the slots 24..31 gather after the
+// first batch), which hides a third more of its memory latency than two
+// waves.
 #ifndef RF_STEP_WAVES_PER_EU
 #define RF_STEP_WAVES_PER_EU 3
 #endif
