@@ -810,15 +810,26 @@ __global__ __launch_bounds__(kBlock) RF_BATCH_ATTR void k_step_batch(
 // (md_results.cu:56-145) on the summed platen forces and bond count, the
 // curve point, and CompressU_S3 / CompressB_S3 (md_data_types.h:347-379,
 // 462-494).
+// Whether `step` records a curve point and bond count (every dnsave steps).
+// A 64-bit remainder is a long software routine; the values met in practice
+// fit 32 bits, where the remainder is the same.
+__device__ __forceinline__ bool sample_step(int64_t step, int64_t dnsave) {
+    const int64_t dn = dnsave > 0 ? dnsave : 1;
+    if (dn == 1) return true;
+    if ((uint64_t)step <= 0xffffffffull && (uint64_t)dn <= 0xffffffffull)
+        return (uint32_t)step % (uint32_t)dn == 0u;
+    return step % dn == 0;
+}
+
 __device__ __forceinline__ void ctrl_logic(const DevState& s, const DevParams& p,
                                            Ctrl* __restrict__ ctrl, const Ctrl& c0, double sum_b,
-                                           double sum_t, long long sum_n, const VwLog* lg) {
+                                           double sum_t, long long sum_n, bool save,
+                                           const VwLog* lg) {
     Ctrl c = c0;
     if (p.kind != RF_TEST_NONE) {
         c.sfl0 = sum_b;
         c.sfl1 = sum_t;
-        const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
-        if (c.step % dn == 0) {
+        if (save) {  // sample_step(c.step, p.dnsave)
             if (c.stepsave < p.curve_cap) {
                 s.bcount[c.stepsave] = sum_n;
                 s.curve[3 * c.stepsave] = (c.size_axis - c.size0) / c.size0;
@@ -889,7 +900,9 @@ struct NoMid {
     __device__ __forceinline__ void operator()(const Ctrl&) const {}
 };
 
-template <class Mid = NoMid>
+// kQ: groups a thread has in flight at once (the registers of 2 kQ double4
+// loads).
+template <int kQ = 4, class Mid = NoMid>
 __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
                                           Ctrl* __restrict__ ctrl, const VwLog* lg = nullptr,
                                           const Mid& mid = Mid()) {
@@ -897,47 +910,74 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     __shared__ double stt[kCtrlThreads];
     __shared__ long long sn[kCtrlThreads];
     const int t = threadIdx.x;
+    const int nblk = p.nblk;
+    const double4* __restrict__ pb = reinterpret_cast<const double4*>(s.part_b);
+    const double4* __restrict__ pt = reinterpret_cast<const double4*>(s.part_t);
     // the partials are read before the halt flag is known (a halted step
     // leaves them as they were), so these loads and the control block's
     // (uniform: scalar loads) are in flight together; nothing in this
     // kernel writes the control block before its last store
     const Ctrl c0 = *ctrl;
-    double ab = 0.0, at = 0.0;
-    long long nb = 0;
+    // Running sum t adds groups t, t + kCtrlThreads, ... in ascending order.
     // Group b's partial: its four waves' entries as (W0 + W2) + (W1 + W3).
-    // Only the groups the step
-    // flagged are read: the others' partials are zeros, and adding a zero
-    // leaves these sums' bits unchanged (they start at +0.0, so they are
-    // never -0.0).
-#pragma unroll 4
-    for (int b = t; b < p.nblk; b += kCtrlThreads) {
-        if ((s.pmask[b >> 5] >> (b & 31)) & 1u) {
-            const double4 vb = reinterpret_cast<const double4*>(s.part_b)[b];
-            const double4 vt = reinterpret_cast<const double4*>(s.part_t)[b];
-            ab = ab + ((vb.x + vb.z) + (vb.y + vb.w));
-            at = at + ((vt.x + vt.z) + (vt.y + vt.w));
+    // Only the groups the step flagged are read: the others' partials are
+    // zeros, and adding a zero leaves these sums' bits unchanged (they start
+    // at +0.0, so they are never -0.0).  The flags of kQ groups per thread
+    // are read together (unconditionally, so that no load waits inside a
+    // branch), then their flagged partials together.
+    double ab = 0.0, at = 0.0;
+    for (int m0 = 0; kCtrlThreads * m0 < nblk; m0 += kQ) {  // uniform
+        uint32_t fm = 0u;  // bit q: group t + kCtrlThreads (m0 + q) flagged
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {  // (past the last group: word 0, no flag)
+            const int b = t + kCtrlThreads * (m0 + q);
+            const int bc = b < nblk ? b : 0;
+            const uint32_t word = s.pmask[bc >> 5];
+            fm |= ((word >> (bc & 31)) & (uint32_t)(b < nblk)) << q;
+        }
+        // all flags in before the first partial load: the in-order load
+        // counter would otherwise make a later flag wait for earlier partials
+        __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
+        double4 vb[kQ], vt[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            if ((fm >> q) & 1u) {
+                const int b = t + kCtrlThreads * (m0 + q);
+                vb[q] = pb[b];
+                vt[q] = pt[b];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            if ((fm >> q) & 1u) {
+                ab = ab + ((vb[q].x + vb[q].z) + (vb[q].y + vb[q].w));
+                at = at + ((vt[q].x + vt[q].z) + (vt[q].y + vt[q].w));
+            }
         }
     }
     // bond counts: only on the steps that record a bond-count sample
-    const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
-    if (p.kind != RF_TEST_NONE && c0.step % dn == 0) {
+    const bool save = sample_step(c0.step, p.dnsave);
+    long long nb = 0;
+    if (p.kind != RF_TEST_NONE && save) {
 #pragma unroll 4
-        for (int b = t; b < p.nblk; b += kCtrlThreads) {
+        for (int b = t; b < nblk; b += kCtrlThreads) {
             const int4 vn = reinterpret_cast<const int4*>(s.part_bonds)[b];
             nb += (vn.x + vn.y) + (vn.z + vn.w);
         }
     }
     __syncthreads();  // every flag read: clear them for the next step
-    for (int w = t; w < (p.nblk + 31) / 32; w += kCtrlThreads) s.pmask[w] = 0u;
+    for (int q = t; q < (nblk + 31) / 32; q += kCtrlThreads) s.pmask[q] = 0u;
     if (c0.halt) return;
     mid(c0);
     // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
-    // one shuffle per level
+    // one shuffle per level.  Sums t >= nblk are +0.0, so a level with
+    // off >= nblk adds +0.0 to values that are never -0.0 and is skipped.
     sb[t] = ab;
     stt[t] = at;
     sn[t] = nb;
     __syncthreads();
     for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
+        if (off >= nblk) continue;  // uniform
         if (t < off) {
             sb[t] = sb[t] + sb[t + off];
             stt[t] = stt[t] + stt[t + off];
@@ -955,7 +995,7 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
         vn += __shfl_down(vn, off);
     }
     if (t != 0) return;
-    ctrl_logic(s, p, ctrl, c0, vb, vt, vn, lg);
+    ctrl_logic(s, p, ctrl, c0, vb, vt, vn, save, lg);
 }
 
 __global__ __launch_bounds__(kCtrlThreads) void k_controller(DevState s, DevParams p,
@@ -974,9 +1014,8 @@ struct SensTangents {
     __device__ __forceinline__ void operator()(const Ctrl& c0) const {
         __shared__ double tb[kCtrlThreads], tt[kCtrlThreads];
         if (p.kind == RF_TEST_NONE) return;
-        const int64_t dn = p.dnsave > 0 ? p.dnsave : 1;
         const int64_t save = c0.stepsave;
-        if (c0.step % dn != 0 || save >= p.curve_cap) return;  // uniform
+        if (!sample_step(c0.step, p.dnsave) || save >= p.curve_cap) return;  // uniform
         const int tid = threadIdx.x;
         for (int k = 0; k < t.nl; ++k) {
             double ab = 0.0, at = 0.0;
@@ -1023,7 +1062,7 @@ __global__ __launch_bounds__(kCtrlThreads) void k_controller_sens(DevState s, De
 // One block per sample.
 __global__ __launch_bounds__(kCtrlThreads) void k_controller_batch(const BatchEntry* __restrict__ e) {
     const BatchEntry& b = e[blockIdx.x];
-    ctrl_body(b.s, b.p, b.ctrl);
+    ctrl_body<2>(b.s, b.p, b.ctrl);  // 2: fewer registers
 }
 
 // ------------------------------------------------------ cell binning ------
