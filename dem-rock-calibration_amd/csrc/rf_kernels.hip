@@ -969,25 +969,36 @@ __device__ __forceinline__ void ctrl_body(const DevState& s, const DevParams& p,
     for (int q = t; q < (nblk + 31) / 32; q += kCtrlThreads) s.pmask[q] = 0u;
     if (c0.halt) return;
     mid(c0);
-    // tree v[t] = v[t] + v[t + off]: through LDS down to one wave, then
-    // one shuffle per level.  Sums t >= nblk are +0.0, so a level with
-    // off >= nblk adds +0.0 to values that are never -0.0 and is skipped.
+    // tree v[t] = v[t] + v[t + off], off = kCtrlThreads / 2 .. 1.  The
+    // levels down to 64 values are formed by wave 0 alone: lane t reads the
+    // 16 sums t + 64 k and adds them in the tree's pairing, so the block
+    // needs one barrier; then one shuffle per level.
+    static_assert(kCtrlThreads == 1024, "the tree below is written for 1024 sums");
     sb[t] = ab;
     stt[t] = at;
     sn[t] = nb;
     __syncthreads();
-    for (int off = kCtrlThreads / 2; off >= 64; off >>= 1) {
-        if (off >= nblk) continue;  // uniform
-        if (t < off) {
-            sb[t] = sb[t] + sb[t + off];
-            stt[t] = stt[t] + stt[t + off];
-            sn[t] += sn[t + off];
-        }
-        __syncthreads();
-    }
     if (t >= 64) return;
-    double vb = sb[t], vt = stt[t];
-    long long vn = sn[t];
+    double xb[16], xt[16];
+    long long xn[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        xb[k] = sb[t + 64 * k];
+        xt[k] = stt[t + 64 * k];
+        xn[k] = sn[t + 64 * k];
+    }
+    // x[k] holds v[t + 64 k]; level off = 64 m pairs x[k] with x[k + m]
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int k = 0; k < m; ++k) {
+            xb[k] = xb[k] + xb[k + m];
+            xt[k] = xt[k] + xt[k + m];
+            xn[k] += xn[k + m];
+        }
+    }
+    double vb = xb[0], vt = xt[0];
+    long long vn = xn[0];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         vb = vb + __shfl_down(vb, off);
