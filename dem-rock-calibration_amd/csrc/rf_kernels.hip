@@ -70,6 +70,14 @@ __device__ __forceinline__ double tree8(double v) {
     return v;
 }
 
+// A value that is the same in every lane, held in scalar registers.
+__device__ __forceinline__ double uniform(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
     const int lo = __shfl((int)(uint32_t)v, src, kLanes);
     const int hi = __shfl((int)(uint32_t)(v >> 32), src, kLanes);
@@ -138,11 +146,11 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, double ri, doub
 #else
 #define RF_STEP_ATTR
 #endif
-// The batched step looks up its sample's arrays and constants per
-// workgroup: held to three waves per SIMD it spills 40 bytes per lane, and
-// measured 3 % slower than with the registers of two waves.
+// The batched step (which looks up its sample's arrays and constants per
+// workgroup) fits the same 168 registers once the platens' positions and
+// speed sit in scalar registers.
 #ifndef RF_BATCH_WAVES_PER_EU
-#define RF_BATCH_WAVES_PER_EU 0
+#define RF_BATCH_WAVES_PER_EU 3
 #endif
 #if RF_BATCH_WAVES_PER_EU > 0
 #define RF_BATCH_ATTR __attribute__((amdgpu_waves_per_eu(RF_BATCH_WAVES_PER_EU)))
@@ -229,7 +237,10 @@ __device__ __forceinline__ void step_body(const DevState& s, const DevParams& p,
     // the halt test (uniform for the whole grid) waits for both.
     const int32_t halt = ctrl->halt;
     const int cur = ctrl->cur;
-    const double top = ctrl->top, bottom = ctrl->bottom, vw = ctrl->vw;
+    // the platens' positions and speed: uniform, kept in scalar registers
+    // until the integration reads them
+    const double top = uniform(ctrl->top), bottom = uniform(ctrl->bottom),
+                 vw = uniform(ctrl->vw);
     const int S = p.S, C = p.C, B = p.B;
     const int64_t ic = valid ? i : 0;  // clamped index for the inactive groups' loads
 

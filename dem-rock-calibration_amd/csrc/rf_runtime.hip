@@ -1314,7 +1314,10 @@ rf_status rf_set_state(rf_ctx* c, const rf_state_view* v) {
 constexpr int kBatchOrders = 5;       // three block orders, one after another, concurrent
 constexpr int kBatchSequential = 3;
 constexpr int kBatchConcurrent = 4;
-constexpr int kTuneSteps = 32;
+// Steps of each form's timed trial chunk: enough that per-chunk host work
+// (64 graph launches and waits in the concurrent form) weighs as it does in
+// the run's longer chunks.
+constexpr int kTuneSteps = 128;
 
 struct rf_batch {
     std::vector<rf_ctx*> c;

@@ -95,7 +95,7 @@ def test_batch_run_equals_single_runs():
     assert b.last_ms() > 0.0
     order, ms = b.tuning()
     print(f"block order {order}, ms per step {ms}", flush=True)
-    assert order in (-1, 0, 1, 2, 3)
+    assert order in (-1, 0, 1, 2, 3, 4)
     if order >= 0:
         assert all(m > 0.0 for m in ms) and ms[order] == min(ms)
     for k, c in enumerate(contexts(ws, skins)):
