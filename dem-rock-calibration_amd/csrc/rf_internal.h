@@ -117,6 +117,14 @@ struct DevParams {
     int64_t zero_idx;    // storage index of the caller's particle 0
 };
 
+// A value that is the same in every lane, held in scalar registers.
+__device__ __forceinline__ double uniform(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Rebuild test.  A candidate rebuild is due once a particle has moved more
 // than skin/2 since the last one.  Only the timing of rebuilds depends on
 // this test, never a result, so the mark is kept in single precision with a

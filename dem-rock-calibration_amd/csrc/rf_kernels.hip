@@ -70,14 +70,6 @@ __device__ __forceinline__ double tree8(double v) {
     return v;
 }
 
-// A value that is the same in every lane, held in scalar registers.
-__device__ __forceinline__ double uniform(double v) {
-    const uint64_t b = __double_as_longlong(v);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
 __device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
     const int lo = __shfl((int)(uint32_t)v, src, kLanes);
     const int hi = __shfl((int)(uint32_t)(v >> 32), src, kLanes);
