@@ -435,7 +435,7 @@ rf_status rf_batch_run(rf_batch* batch, rf_test_state* out);
 /* Wall time (ms) of the last rf_batch_step / rf_batch_run. */
 rf_status rf_batch_last_ms(rf_batch* batch, double* total_ms);
 /* The batch's launch autotuner (no reference counterpart): the first steps of
- * a batch run in short chunks in each of five ways -- one shared launch per
+ * a batch run in 128-step chunks in each of five ways -- one shared launch per
  * step with the blocks ordered by sample (0), interleaved (1) or largest
  * sample first (2), each sample stepped alone, one after another (3), or
  * each sample on its own stream with all chunks launched before any is
