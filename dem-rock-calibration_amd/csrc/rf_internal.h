@@ -36,6 +36,8 @@ constexpr int kErrSensList = 16;     // a sensitivity window would change the sl
 
 // Parameter directions carried by one sensitivity run.
 constexpr int kSensK = 4;
+// Waves of the sensitivity step a CU holds at once: one per SIMD (its registers).
+constexpr int kSensWavesPerCU = 4;
 
 // One bond of a particle's bond table: 128 bytes, the 12 history values
 // first (written back each step), then the constants.
@@ -163,7 +165,8 @@ struct Ctrl {
 // bond radius) along direction k.
 struct SensState {
     int32_t nl;        // lanes per particle = parameter directions in use
-    int32_t pad_;
+    int32_t nwaves;    // waves of a sensitivity step
+    int32_t* wbase;    // wave w steps particles wbase[w] .. wbase[w + 1] - 1
     double* pp;        // per particle: bottom, top platen force (values)
     int32_t* pnb;      // per particle: bonds left
     double* tpp;       // per particle and direction: bottom, top tangents

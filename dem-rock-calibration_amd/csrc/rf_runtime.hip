@@ -1766,6 +1766,50 @@ rf_status material_tangent(const rf_material* m, double wall_factor, int32_t par
     return RF_OK;
 }
 
+// The particles of each wave of the sensitivity step.  A wave walks its
+// particles' bonds over 64 / nl lanes per parameter direction, so its time
+// follows ceil(bond entries / lanes).  Runs of consecutive particles (at most
+// one per lane) are cut where the entries would exceed `target` per lane,
+// with the smallest target whose runs the device holds at once; bond-dense
+// regions then get shorter runs, and their lanes without a particle take
+// bond positions handed out by the others.  The runs only balance the waves:
+// every particle's sums keep their order whatever run it falls in.
+rf_status sens_wave_table(rf_ctx* c) {
+    rf::SensState& t = c->sens;
+    const int64_t n = c->n;
+    const int ppw = 64 / t.nl, B = c->B;
+    std::vector<int32_t> nb((size_t)n);
+    HIP_TRY(hipMemcpy(nb.data(), c->d.nbond, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int64_t resident = (int64_t)std::max(cus, 1) * rf::kSensWavesPerCU;
+    int64_t total = 0;
+    for (int64_t q = 0; q < n; ++q) total += std::min(nb[q], B);
+    std::vector<int32_t> wb;
+    int64_t target = std::max<int64_t>(1, (total + resident * ppw - 1) / (resident * ppw));
+    for (;; ++target) {
+        wb.assign(1, 0);
+        int cnt = 0;
+        int64_t ent = 0;
+        for (int64_t q = 0; q < n; ++q) {
+            const int x = std::min(nb[q], B);
+            if (cnt == ppw || (cnt > 0 && ent + x > ppw * target)) {
+                wb.push_back((int32_t)q);
+                cnt = 0;
+                ent = 0;
+            }
+            ++cnt;
+            ent += x;
+        }
+        wb.push_back((int32_t)n);
+        // at target B every run is ppw particles long: no shorter runs exist
+        if ((int64_t)wb.size() - 1 <= resident || target >= B) break;
+    }
+    t.nwaves = (int32_t)(wb.size() - 1);
+    HIP_TRY(hipMemcpy(t.wbase, wb.data(), wb.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return RF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1815,6 +1859,7 @@ rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
         if (!st) st = c->alloc(&t.tpp, 2 * K * n);
         if (!st) st = c->alloc(&t.tpart_b, K * c->p.nblk);
         if (!st) st = c->alloc(&t.tpart_t, K * c->p.nblk);
+        if (!st) st = c->alloc(&t.wbase, n + 2);
         if (st) return st;
         c->sens_on = true;
     }
@@ -1844,6 +1889,8 @@ rf_status rf_sens_begin(rf_ctx* c, int32_t nparam, const int32_t* params) {
         }
     }
     t.nl = nparam;
+    st = sens_wave_table(c);
+    if (st) return st;
     t.br_step0 = c->hctrl.step;
     c->sens_np = nparam;
     c->sens_curve0 = c->hctrl.stepsave;

@@ -203,12 +203,22 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
         return;
     }
     const int nl = t.nl;
-    const int ppw = 64 / nl;  // particles per wave
+    const int ppw = 64 / nl;  // class lanes per wave (at most this many particles)
     const int lane = threadIdx.x & 63;
     const int dir = lane % nl, pw = lane / nl;
     const bool lead = dir == 0;
-    const int64_t i = ((int64_t)blockIdx.x * (kSensBlock / 64) + (threadIdx.x >> 6)) * ppw + pw;
-    const bool valid = pw < ppw && i < p.n;
+    // the wave's particles: a run of at most ppw from the table the run's
+    // start laid out (runs in bond-dense regions are shorter)
+    const int wv = threadIdx.x >> 6;
+    const int64_t gw = (int64_t)blockIdx.x * kSensWaves + wv;
+    if (gw >= t.nwaves) return;  // wave-uniform
+    const int64_t wave_base = t.wbase[gw];
+    const int wcount = (int)(t.wbase[gw + 1] - wave_base);
+    const int64_t i = wave_base + pw;
+    const bool valid = pw < wcount;
+    // a class lane without a particle in this wave only takes bond positions
+    // handed out by the others
+    const bool helper = !valid && pw < ppw;
     const int cur = ctrl->cur, nxt = cur ^ 1;
     const double top = ctrl->top, bottom = ctrl->bottom, vw = ctrl->vw;
     const int S = p.S, C = p.C, B = p.B;
@@ -217,9 +227,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
     int nb_out = 0;
 
     // ---- bond balancing within each direction class (every lane of the wave)
-    const int64_t ic = valid ? i : 0;
-    const int wv = threadIdx.x >> 6;
-    const int64_t wave_base = ((int64_t)blockIdx.x * kSensWaves + wv) * ppw;
+    const int64_t ic = valid ? i : 0;  // the particle (0 for the other lanes' loads)
     const uint32_t bnd0 = valid ? s.bnd[ic] : 0u;
     const int nbd0 = valid ? s.nbond[ic] : 0;
     const int nown = nbd0 < B ? nbd0 : B;
@@ -243,12 +251,12 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
     };
     const uint32_t rec3_0 =
         MODE == 2 && valid ? t.br_bits[(ls * p.n + ic) * kBrWords + 3] : 0u;
-    const int last = (ppw - 1) * nl + dir;  // the class's last particle lane
+    const int last = (ppw - 1) * nl + dir;  // the class's last lane
     const int ntot = __shfl(class_scan(nown, pw, nl), last);
-    const int nval = __shfl(class_scan(valid ? 1 : 0, pw, nl), last);
-    int target = nval > 0 ? (ntot + nval - 1) / nval : 0;
+    // every lane of the class walks at most `target` positions
+    int target = (ntot + ppw - 1) / ppw;
     int excess = nown > target ? nown - target : 0;
-    int freec = valid && target > nown ? target - nown : 0;
+    int freec = (valid || helper) && target > nown ? target - nown : 0;
     const int ex_in = class_scan(excess, pw, nl), fr_in = class_scan(freec, pw, nl);
     int nhelp = __shfl(ex_in, last);  // the same in every class (same particles)
     int ex_pre = ex_in - excess, fr_pre = fr_in - freec;
@@ -282,21 +290,21 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
-    if (valid) {
+    if (valid || helper) {  // helpers: the bond walk only
         const double4* pos = s.pos[cur];
         const double4* vel = s.vel[cur];
         const double4* angv = s.angv[cur];
-        Vec4<D> pi = load4(pos[i], t.tpos[cur], i, dir);
-        Vec4<D> vi = load4(vel[i], t.tvel[cur], i, dir);
-        Vec4<D> wi = load4(angv[i], t.tang[cur], i, dir);
-        uint32_t occ = s.occ[i], bnd = bnd0, otnz = s.otnz[i];
-        uint32_t* const lw = MODE != 0 ? t.br_bits + (ls * p.n + i) * kBrWords : nullptr;
+        Vec4<D> pi = load4(pos[ic], t.tpos[cur], ic, dir);
+        Vec4<D> vi = load4(vel[ic], t.tvel[cur], ic, dir);
+        Vec4<D> wi = load4(angv[ic], t.tang[cur], ic, dir);
+        uint32_t occ = valid ? s.occ[ic] : 0u, bnd = bnd0, otnz = valid ? s.otnz[ic] : 0u;
+        uint32_t* const lw = MODE != 0 ? t.br_bits + (ls * p.n + ic) * kBrWords : nullptr;
         uint32_t rec[kBrWords] = {0u, 0u, 0u, 0u, 0u};  // the decisions (taken or replayed)
         double4 wrec = make_double4(0.0, 0.0, 0.0, 0.0);
         if (MODE == 2) {
 #pragma unroll
             for (int u = 0; u < kBrWords; ++u) rec[u] = lw[u];
-            wrec = t.br_w[ls * p.n + i];
+            wrec = t.br_w[ls * p.n + ic];
         }
         // The slot table is held fixed.  Contact pairs that cross acut
         // (> r_i + r_j) carry no force, so leaving them in or out changes
@@ -317,10 +325,10 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int k = 8 * j + u;
-                jn[u] = s.nbr[i * S + (k < S ? k : S - 1)];
+                jn[u] = s.nbr[ic * S + (k < S ? k : S - 1)];
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = pos[((blk >> u) & 1u) ? jn[u] : (int32_t)i];
+            for (int u = 0; u < 8; ++u) q[u] = pos[((blk >> u) & 1u) ? jn[u] : (int32_t)ic];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 if (!((blk >> u) & 1u)) continue;
@@ -330,7 +338,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 if (::sqrt(rr) < pi.w.v + q[u].w) ovl_l |= 1u << (8 * j + u);
             }
         }
-        const int ncd = s.ncand[i] < C ? s.ncand[i] : C;
+        const int ncd = !valid ? 0 : s.ncand[ic] < C ? s.ncand[ic] : C;
         for (int j = dir; 8 * j < ncd; j += nl) {
             int32_t jc[8];
             int cs[8];
@@ -338,13 +346,13 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int c = 8 * j + u;
-                const int64_t qc = i * C + (c < C ? c : C - 1);
+                const int64_t qc = ic * C + (c < C ? c : C - 1);
                 jc[u] = s.cand[qc];
                 cs[u] = s.cslot[qc];
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                q[u] = pos[(8 * j + u < ncd && cs[u] < 0) ? jc[u] : (int32_t)i];
+                q[u] = pos[(8 * j + u < ncd && cs[u] < 0) ? jc[u] : (int32_t)ic];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 if (!(8 * j + u < ncd && cs[u] < 0)) continue;
@@ -410,7 +418,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
             bool alive;
             uint32_t brk_in;
             if (own) {
-                ip = i;
+                ip = ic;
                 b = seq_bond(r, nown);
                 slot_partner(b, ks, j);
                 alive = ks >= 0 && ((bnd >> ks) & 1u);
@@ -511,8 +519,8 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
 #pragma unroll 1
             for (int k = l; k < 32; k += kLanes) {
                 if (!((ovl >> k) & 1u)) continue;
-                const int64_t j = s.nbr[i * S + k];
-                const int64_t qs = i * S + k;
+                const int64_t j = s.nbr[ic * S + k];
+                const int64_t qs = ic * S + k;
                 const double4 pjv = pos[j];
                 const Vec4<D> pj = load4(pjv, t.tpos[cur], j, dir);
                 const Vec4<D> vj = load4(vel[j], t.tvel[cur], j, dir);
@@ -528,7 +536,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                 br.wx = wrec.x;
                 br.wy = wrec.y;
                 br.wz = wrec.z;
-                contact_eval_t<D>(m, p, pi, vi, wi, pj, vj, wj, op, wim, roll, s.pk[i].y, c6, on,
+                contact_eval_t<D>(m, p, pi, vi, wi, pj, vj, wj, op, wim, roll, s.pk[ic].y, c6, on,
                                   br);
                 if (MODE == 1) {
                     rec[1] |= ((br.out >> kBrSlip) & 1u) << k;
@@ -557,7 +565,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
             M[u] = (M[u] + bsum[3 + u]) + csum[3 + u];
         }
         // platens
-        if (p.kind != RF_TEST_NONE) {
+        if (valid && p.kind != RF_TEST_NONE) {
             D px_, py_, pz_, r2;
             if (p.kind == RF_TEST_BRAZILIAN) {
                 px_ = pi.x - p.scope_c[0]; py_ = pi.y; pz_ = pi.z - p.scope_c[2];
@@ -588,7 +596,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                     const double nwy = ax == 1 ? 1.0 : 0.0, nwz = ax == 2 ? 1.0 : 0.0;
                     const double wvy = ax == 1 ? vw : 0.0, wvz = ax == 2 ? vw : 0.0;
                     wall_contact_t<D>(m, p, ri, mi, vi, wi, 0.0, nwy, nwz, 0.0, wvy, wvz,
-                                      fabs(top - ppos), o0, o1, o2, s.pk[i].y, wo, br);
+                                      fabs(top - ppos), o0, o1, o2, s.pk[ic].y, wo, br);
                     if (MODE == 1)
                         rec[4] |= kBfTop | ((br.out >> kBrSlip) & 1u ? kBfTopSlip : 0u) |
                                   ((br.out >> kBrChat) & 1u ? kBfTopChat : 0u);
@@ -606,7 +614,7 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
                     const double nwy = ax == 1 ? -1.0 : 0.0, nwz = ax == 2 ? -1.0 : 0.0;
                     const double wvy = ax == 1 ? -vw : 0.0, wvz = ax == 2 ? -vw : 0.0;
                     wall_contact_t<D>(m, p, ri, mi, vi, wi, 0.0, nwy, nwz, 0.0, wvy, wvz,
-                                      fabs(bottom - ppos), o0, o1, o2, s.pk[i].y, wo, br);
+                                      fabs(bottom - ppos), o0, o1, o2, s.pk[ic].y, wo, br);
                     if (MODE == 1)
                         rec[4] |= kBfBot | ((br.out >> kBrSlip) & 1u ? kBfBotSlip : 0u) |
                                   ((br.out >> kBrChat) & 1u ? kBfBotChat : 0u);
@@ -619,34 +627,36 @@ __global__ __launch_bounds__(kSensBlock, RF_SENS_WAVES) void k_sens_step(DevStat
             pt = -s_t;
         }
         // kick-drift; the window must not lose particles
-        const double2 pk = s.pk[i];
+        const double2 pk = s.pk[ic];
         const D nvx = vi.x + F[0] * pk.x, nvy = vi.y + F[1] * pk.x, nvz = vi.z + F[2] * pk.x;
         const D nrx = pi.x + nvx * p.dt, nry = pi.y + nvy * p.dt, nrz = pi.z + nvz * p.dt;
         const D nwx = wi.x + M[0] * pk.y, nwy = wi.y + M[1] * pk.y, nwz = wi.z + M[2] * pk.y;
-        t.tpos[nxt][i * kSensK + dir] = make_double4(nrx.d[0], nry.d[0], nrz.d[0], 0.0);
-        t.tvel[nxt][i * kSensK + dir] = make_double4(nvx.d[0], nvy.d[0], nvz.d[0], 0.0);
-        t.tang[nxt][i * kSensK + dir] = make_double4(nwx.d[0], nwy.d[0], nwz.d[0], 0.0);
-        if (lead) {
+        if (valid) {
+            t.tpos[nxt][ic * kSensK + dir] = make_double4(nrx.d[0], nry.d[0], nrz.d[0], 0.0);
+            t.tvel[nxt][ic * kSensK + dir] = make_double4(nvx.d[0], nvy.d[0], nvz.d[0], 0.0);
+            t.tang[nxt][ic * kSensK + dir] = make_double4(nwx.d[0], nwy.d[0], nwz.d[0], 0.0);
+        }
+        if (lead && valid) {
             const double dx = nrx.v - p.cen[0], dy = nry.v - p.cen[1], dz = nrz.v - p.cen[2];
             if (dx - p.half[0] > 0.0 || dx + p.half[0] < 0.0 || dy - p.half[1] > 0.0 ||
                 dy + p.half[1] < 0.0 || dz - p.half[2] > 0.0 || dz + p.half[2] < 0.0)
                 atomicOr(&ctrl->err, kErrSensList);
-            s.pos[nxt][i] = make_double4(nrx.v, nry.v, nrz.v, pi.w.v);
-            s.vel[nxt][i] = make_double4(nvx.v, nvy.v, nvz.v, vi.w.v);
-            s.angv[nxt][i] = make_double4(nwx.v, nwy.v, nwz.v, 0.0);
+            s.pos[nxt][ic] = make_double4(nrx.v, nry.v, nrz.v, pi.w.v);
+            s.vel[nxt][ic] = make_double4(nvx.v, nvy.v, nvz.v, vi.w.v);
+            s.angv[nxt][ic] = make_double4(nwx.v, nwy.v, nwz.v, 0.0);
             if (p.store_fm) {
-                s.F[i] = make_double4(F[0].v, F[1].v, F[2].v, 0.0);
-                s.M[i] = make_double4(M[0].v, M[1].v, M[2].v, 0.0);
+                s.F[ic] = make_double4(F[0].v, F[1].v, F[2].v, 0.0);
+                s.M[ic] = make_double4(M[0].v, M[1].v, M[2].v, 0.0);
             }
-            s.occ[i] = occ;
-            s.bnd[i] = bnd;
-            s.otnz[i] = otnz;
+            s.occ[ic] = occ;
+            s.bnd[ic] = bnd;
+            s.otnz[ic] = otnz;
             if (MODE == 1) {
 #pragma unroll
                 for (int u = 0; u < kBrWords; ++u) lw[u] = rec[u];
-                t.br_w[ls * p.n + i] = wrec;
+                t.br_w[ls * p.n + ic] = wrec;
             }
-            if (rebuild_due(s.rbuild[i], nrx.v, nry.v, nrz.v)) ctrl->rebuild = 1;
+            if (rebuild_due(s.rbuild[ic], nrx.v, nry.v, nrz.v)) ctrl->rebuild = 1;
         }
         nb_out = __popc(bnd);
     }
@@ -714,8 +724,7 @@ __global__ __launch_bounds__(256) void k_sens_partials(DevParams p, SensState t,
 hipError_t launch_sens_step(const DevState& s, const DevParams& p, const SensState& t, Ctrl* ctrl,
                             hipStream_t st) {
     if (p.B != kMaxBonds) return hipErrorInvalidValue;  // bond rows read 16 entries at once
-    const int64_t per_block = (int64_t)(kSensBlock / 64) * (64 / t.nl);
-    const dim3 grid((unsigned)((p.n + per_block - 1) / per_block));
+    const dim3 grid((unsigned)((t.nwaves + kSensWaves - 1) / kSensWaves));
     if (t.br_mode == 1)
         hipLaunchKernelGGL(k_sens_step<1>, grid, dim3(kSensBlock), 0, st, s, p, t, ctrl);
     else if (t.br_mode == 2)
