@@ -29,8 +29,9 @@ HBM_PEAK_GBPS = 8000.0
 # timestamps (hipExtLaunchKernelGGL on the context's stream).
 TIMING_STRIDE = 8
 # Measured HBM traffic of one step-kernel launch (rocprofv3 FETCH_SIZE and
-# WRITE_SIZE, scaled by a copy kernel of known size), for the default workload.
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_step_uniaxial_100k_pmc.md")
+# WRITE_SIZE, scaled by a copy kernel of known size), one summary per
+# workload measured.
+PMC_SUMMARIES = os.path.join(ROOT, "profiles", "k_step_*_pmc.md")
 
 
 def log(msg):
@@ -57,20 +58,23 @@ def step_kernel_bytes(c):
 
 
 def measured_traffic(whash):
-    """HBM bytes per step-kernel launch from the committed PMC summary, when
-    it was measured on this very workload (same hash); else None."""
-    try:
-        with open(PMC_SUMMARY) as f:
-            lines = f.read().splitlines()
-    except OSError:
-        return None
-    vals = {}
-    for ln in lines:
-        if ":" in ln and ln.split(":")[0].strip() in ("workload_hash", "traffic_bytes_per_launch"):
-            k, v = ln.split(":", 1)
-            vals[k.strip()] = v.strip().strip("`")
-    if vals.get("workload_hash") == whash and "traffic_bytes_per_launch" in vals:
-        return float(vals["traffic_bytes_per_launch"])
+    """HBM bytes per step-kernel launch from a committed PMC summary measured
+    on this very workload (same hash); else None."""
+    import glob
+    for path in sorted(glob.glob(PMC_SUMMARIES)):
+        try:
+            with open(path) as f:
+                lines = f.read().splitlines()
+        except OSError:
+            continue
+        vals = {}
+        for ln in lines:
+            if ":" in ln and ln.split(":")[0].strip() in ("workload_hash",
+                                                         "traffic_bytes_per_launch"):
+                k, v = ln.split(":", 1)
+                vals[k.strip()] = v.strip().strip("`")
+        if vals.get("workload_hash") == whash and "traffic_bytes_per_launch" in vals:
+            return float(vals["traffic_bytes_per_launch"])
     return None
 
 
